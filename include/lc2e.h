@@ -1,0 +1,157 @@
+/*
+ * lc2e.h -- C ABI of the MI355X LC2 similarity engine.
+ *
+ * Plain C types only (no HIP or torch types): opaque handles, pointers and sizes.
+ * Every entry point returns an int status: LC2E_OK (0) or LC2E_OK_DEGENERATE (1)
+ * on success, a negative LC2E_ERR_* code on failure; lc2e_last_error() gives the
+ * message of the last failure on the calling thread. No C++ exception crosses it.
+ * Host memory passed in is caller-owned and only read/written during the call
+ * (the *_async entry point excepted: its device buffers must stay alive until
+ * the stream reaches the work). Device memory is engine-owned, resident until
+ * the matching *_destroy. One engine serves one host thread at a time.
+ *
+ * The reference keeps this path behind C++ classes (Interpolator*, LC2_*_class)
+ * and CUDA __constant__ state; each entry point below names the reference
+ * interface it replaces (file:line, relative to the reference repository).
+ */
+#ifndef LC2E_H
+#define LC2E_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LC2E_OK 0
+#define LC2E_OK_DEGENERATE 1     /* LC2 was NaN/Inf (sum of weights 0) and is returned as 0.0 */
+#define LC2E_ERR_INVALID (-1)    /* bad argument or handle */
+#define LC2E_ERR_HIP (-2)        /* HIP runtime failure */
+#define LC2E_ERR_BUDGET (-3)     /* allocation would exceed the engine's HBM budget */
+#define LC2E_ERR_UNSUPPORTED (-4)
+#define LC2E_ERR_SINGULAR (-5)   /* singular transform */
+
+typedef struct lc2e_engine lc2e_engine;
+typedef struct lc2e_volume lc2e_volume;
+typedef struct lc2e_pair lc2e_pair;
+
+typedef enum {
+    LC2E_U8 = 0,
+    LC2E_I16 = 1,
+    LC2E_F16 = 2,
+    LC2E_F32 = 3,
+    LC2E_STORE_AUTO = 100 /* storage only: smallest lossless of F16 / F32 */
+} lc2e_dtype;
+
+/* x-fastest layout: idx = x + y*dims[0] + z*dims[0]*dims[1]; 2D images use dims[2] = 1,
+ * a stack of 2D frames uses dims[2] = number of frames. */
+typedef struct {
+    int dims[3];
+    float spacing[3];
+    float origin[3];
+} lc2e_geom;
+
+typedef enum {
+    LC2E_STAGE_WARP_I = 0,   /* high-resolution image on the virtual grid, float32 */
+    LC2E_STAGE_WARP_US = 1,  /* ultrasound image on the virtual grid, float32 */
+    LC2E_STAGE_GRAD = 2,     /* |Sobel(I)|, float32 */
+    LC2E_STAGE_W = 3,        /* per-voxel weight w = std(US patch), float64 */
+    LC2E_STAGE_MEASURE = 4   /* per-voxel measure, float64 */
+} lc2e_stage;
+
+/* Static description of a pair's virtual grid and the constants the pose ->
+ * matrix conversion needs (Interpolator*::InitiateMappers, interpolator.cu:495-529). */
+typedef struct {
+    lc2e_geom virt;          /* origin = fixed origin, spacing = resampled, dims per ceil rule */
+    float fixed_center[3];   /* rotation centre = fixed image centre */
+    float shift[3];          /* c_moving - c_virtual if center_overlaid, else 0 */
+    int grid_ndim;           /* 2 or 3 */
+    int src_ndim;            /* 2 (bilinear 2D moving image) or 3 (trilinear volume) */
+    int high_res_is_moving;
+    int patch_radius;
+    int n_frames;            /* planes of an independent-2D-frame stack (1 otherwise) */
+    int per_frame_matrices;  /* 1: each pose carries n_frames matrices */
+} lc2e_pair_info;
+
+/* Last error message of the calling thread ("" if none). */
+const char* lc2e_last_error(void);
+
+/* Engine on one device with an HBM budget for everything it allocates (0 = no limit).
+ * Replaces the reference's implicit default-device/default-stream state
+ * (interpolator.cuh:27-41 global __constant__ mappers). */
+int lc2e_create(int device, size_t hbm_budget_bytes, lc2e_engine** out);
+void lc2e_destroy(lc2e_engine* e);
+/* Bytes currently allocated by the engine and its budget. */
+int lc2e_memory(lc2e_engine* e, size_t* used, size_t* budget);
+
+/* Upload a resident fixed or moving image. Replaces Interpolator*'s thrust copies of the
+ * volumes (interpolator.cu:276, :315; :1616-1627 for raw files). `storage` is LC2E_F16,
+ * LC2E_F32 or LC2E_STORE_AUTO (U8 -> F16, F16 -> F16, I16/F32 -> F32: all lossless). */
+int lc2e_volume_upload(lc2e_engine* e, const void* host, lc2e_dtype dtype, const lc2e_geom* geom,
+                       lc2e_dtype storage, lc2e_volume** out);
+void lc2e_volume_destroy(lc2e_volume* v);
+
+/* A registration pair: virtual grid (interpolator.cu:320, :337-339), centre and shift
+ * (InitiateMappers, interpolator.cu:495-529), LC2 buffers (LC2_*_class::PortedFromInterpolator,
+ * cu_lc2_3d.cu:942-975) and the pose-independent warp of the fixed image.
+ *  high_res_is_moving: HighResolutionModality "Moving" (1) or "Fixed" (0), interpolator.cu:590-605.
+ *  dim: LC2 grid dimensionality; 2 with a 3D moving volume is the 2D->3D mode
+ *       (Interpolator2DTo3D, interpolator.cu:1538-1786); a fixed image with dims[2] > 1 and
+ *       dim 2 is a stack of independent 2D frames. */
+int lc2e_pair_create(lc2e_engine* e, lc2e_volume* fixed, lc2e_volume* moving,
+                     const float resampled_spacing[3], int high_res_is_moving,
+                     int center_overlaid, int patch_radius, int dim, lc2e_pair** out);
+void lc2e_pair_destroy(lc2e_pair* p);
+int lc2e_pair_get_info(lc2e_pair* p, lc2e_pair_info* info);
+/* Frame stacks only: 1 = every pose carries one matrix per frame (tracked sweeps). */
+int lc2e_pair_set_per_frame(lc2e_pair* p, int per_frame_matrices);
+
+/* Pose parameters -> VirtualToMoving 3x4 row-major float matrix of this pair
+ * (Spatial3DMapper::AffineInit/RigidInit spatialMappers.cuh:579-850,
+ *  Spatial2DMapper::AffineInit/RigidInit :977-1033).
+ *  3D sources: params = [TX,TY,TZ,RX,RY,RZ,SX,SY,SZ,ShX,ShY,ShZ] (n = 12) or rigid (n = 6).
+ *  2D sources: params = [TX,TY,R,SX,SY,Sh] (n = 6) or rigid (n = 3); the 2x3 matrix is
+ *  embedded as [[a,b,0,tx],[c,d,0,ty],[0,0,1,0]]. */
+int lc2e_pose_to_matrix(lc2e_pair* p, const double* params, int n_params, float out12[12]);
+
+/* One pose: GenerateMappers -> Interpolate -> GetSimilarityMeasure
+ * (interpolator.cu:542-631; LC2_*_class::GetSimilarityMetric cu_lc2_3d.cu:1119-1194,
+ * cu_lc2_2d.cu:599-653). m12 = VirtualToMoving as 3x4 row-major (n_frames*12 floats for
+ * per-frame stacks). LC2 NaN/Inf -> 0 with status LC2E_OK_DEGENERATE. */
+int lc2e_eval(lc2e_pair* p, const float* m12, double* lc2);
+
+/* n poses in one fused launch; no counterpart in the reference, whose global __constant__
+ * transform allows one pose in flight (interpolator.cuh:27-41). Results are bit-identical
+ * to n single evaluations. Optional outputs (NULL to skip): sum of measures, sum of w. */
+int lc2e_eval_batch(lc2e_pair* p, int n, const float* mats, double* lc2, double* sum_measure,
+                    double* sum_w);
+
+/* Stream-ordered form for serving loops: d_mats and d_out are DEVICE pointers,
+ * d_out receives 3 doubles per pose (sum_measure, sum_w, lc2). `stream` is a hipStream_t
+ * (NULL = the engine's stream). Nothing is synchronised. */
+int lc2e_eval_batch_async(lc2e_pair* p, int n, const float* d_mats, double* d_out, void* stream);
+int lc2e_sync(lc2e_engine* e);
+
+/* Per-stage buffers for parity tests at one pose (the reference's buffer getters,
+ * cu_lc2_3d.cuh:45-49, cu_lc2_2d.cuh:40-46). host receives virt voxels of float32
+ * (WARP_I, WARP_US, GRAD) or float64 (W, MEASURE). */
+int lc2e_debug_fetch(lc2e_pair* p, lc2e_stage stage, const float* m12, void* host);
+
+/* Launch configuration of the fused kernel (z planes per work-group; 0 = heuristic). */
+int lc2e_pair_set_tuning(lc2e_pair* p, int zchunk);
+int lc2e_pair_get_tuning(lc2e_pair* p, int* zchunk);
+
+/* Kernel timing with HIP events recorded around every fused-kernel launch on the engine's
+ * stream (eager path only). enable = 1 resets the counters. read returns the summed
+ * fused-kernel time in ms and the number of launches since the reset. */
+int lc2e_pair_profile(lc2e_pair* p, int enable);
+int lc2e_pair_profile_read(lc2e_pair* p, double* kernel_ms, long long* launches);
+
+/* Record the batched evaluation of n poses in a HIP graph and replay it for later
+ * lc2e_eval_batch calls with the same n (1 = on, 0 = off). */
+int lc2e_pair_use_graph(lc2e_pair* p, int enable);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

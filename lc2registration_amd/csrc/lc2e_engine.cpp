@@ -1,0 +1,619 @@
+// C ABI of the LC2 engine (include/lc2e.h): resident volumes, pairs, batched
+// evaluation on one HIP stream, HBM budget accounting, optional HIP-graph replay.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "lc2e.h"
+#include "lc2e_geometry.hpp"
+#include "lc2e_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess)                                                              \
+            return fail(LC2E_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                        __FILE__, __LINE__);                                               \
+    } while (0)
+
+int dtype_size(int dt) {
+    switch (dt) {
+        case LC2E_U8: return 1;
+        case LC2E_I16: return 2;
+        case LC2E_F16: return 2;
+        case LC2E_F32: return 4;
+        default: return 0;
+    }
+}
+
+}  // namespace
+
+struct lc2e_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t budget = 0;  // 0 = unlimited
+    size_t used = 0;
+
+    int alloc(void** p, size_t bytes) {
+        if (budget && used + bytes > budget)
+            return fail(LC2E_ERR_BUDGET, "HBM budget exceeded: %zu used + %zu requested > %zu",
+                        used, bytes, budget);
+        HIP_TRY(hipSetDevice(device));
+        HIP_TRY(hipMalloc(p, bytes ? bytes : 1));
+        used += bytes;
+        return LC2E_OK;
+    }
+    void release(void* p, size_t bytes) {
+        if (p) {
+            (void)hipFree(p);
+            used -= bytes;
+        }
+    }
+};
+
+struct lc2e_volume {
+    lc2e_engine* e = nullptr;
+    void* d = nullptr;
+    size_t bytes = 0;
+    int f16 = 0;
+    lc2e_geom g{};
+};
+
+struct GraphEntry {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+struct lc2e_pair {
+    lc2e_engine* e = nullptr;
+    lc2e_volume* fixed = nullptr;
+    lc2e_volume* moving = nullptr;
+    lc2e_pair_info info{};
+    lc2e::VGeom v{};
+    lc2e::SGeom sg{};
+    float* stat_img = nullptr;
+    float* stat_grad = nullptr;
+    size_t vbytes = 0;
+    int zchunk = 0;
+    int ntx = 0, nty = 0, nzc = 0, blocks_per_pose = 0;
+    // batch buffers
+    int cap = 0;
+    float* d_mats = nullptr;
+    double* d_partials = nullptr;
+    double* d_out = nullptr;
+    float* h_mats = nullptr;   // pinned
+    double* h_out = nullptr;   // pinned
+    size_t cap_bytes = 0;
+    bool use_graph = false;
+    std::map<int, GraphEntry> graphs;
+    // kernel timing (eager path)
+    bool profile = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double prof_ms = 0.0;
+    long long prof_launches = 0;
+
+    int mats_per_pose() const { return info.per_frame_matrices ? info.n_frames : 1; }
+};
+
+namespace {
+
+void drop_graphs(lc2e_pair* p) {
+    for (auto& kv : p->graphs) {
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+    }
+    p->graphs.clear();
+}
+
+int heuristic_zchunk(const lc2e_pair* p) {
+    // Enough work-groups per pose to cover 256 CUs x 2 resident groups for a single pose,
+    // while keeping the z halo overhead (2R+2 extra planes) small.
+    if (p->info.grid_ndim != 3) return 1;
+    const int r = p->info.patch_radius;
+    int ntx, nty, nzc;
+    const int cols = (lc2e::fused_blocks_3d(p->v, r, 1 << 20, &ntx, &nty, &nzc) > 0) ? ntx * nty : 1;
+    const int ez = p->v.Z - 2 * r;
+    int zc = 32;
+    while (zc > 8 && cols * ((ez + zc - 1) / zc) < 1024) zc /= 2;
+    return zc;
+}
+
+int update_tiling(lc2e_pair* p) {
+    if (p->info.grid_ndim == 3) {
+        if (p->zchunk <= 0) p->zchunk = heuristic_zchunk(p);
+        p->blocks_per_pose = lc2e::fused_blocks_3d(p->v, p->info.patch_radius, p->zchunk, &p->ntx,
+                                                   &p->nty, &p->nzc);
+    } else {
+        p->zchunk = 1;
+        p->blocks_per_pose = lc2e::fused_blocks_2d(p->v, p->info.patch_radius, &p->ntx, &p->nty);
+        p->nzc = p->v.Z;
+    }
+    return LC2E_OK;
+}
+
+void free_batch(lc2e_pair* p) {
+    drop_graphs(p);
+    lc2e_engine* e = p->e;
+    if (p->d_mats) e->release(p->d_mats, (size_t)p->cap * p->mats_per_pose() * 12 * sizeof(float));
+    if (p->d_partials)
+        e->release(p->d_partials, (size_t)p->cap * (p->blocks_per_pose > 0 ? p->blocks_per_pose : 1) *
+                                      2 * sizeof(double));
+    if (p->d_out) e->release(p->d_out, (size_t)p->cap * 3 * sizeof(double));
+    if (p->h_mats) (void)hipHostFree(p->h_mats);
+    if (p->h_out) (void)hipHostFree(p->h_out);
+    p->d_mats = nullptr;
+    p->d_partials = nullptr;
+    p->d_out = nullptr;
+    p->h_mats = nullptr;
+    p->h_out = nullptr;
+    p->cap = 0;
+}
+
+int ensure_capacity(lc2e_pair* p, int n) {
+    if (n <= p->cap) return LC2E_OK;
+    free_batch(p);
+    int cap = 1;
+    while (cap < n) cap *= 2;
+    lc2e_engine* e = p->e;
+    const int mpp = p->mats_per_pose();
+    const int bpp = p->blocks_per_pose > 0 ? p->blocks_per_pose : 1;
+    int rc;
+    if ((rc = e->alloc((void**)&p->d_mats, (size_t)cap * mpp * 12 * sizeof(float))) < 0) return rc;
+    if ((rc = e->alloc((void**)&p->d_partials, (size_t)cap * bpp * 2 * sizeof(double))) < 0) {
+        e->release(p->d_mats, (size_t)cap * mpp * 12 * sizeof(float));
+        p->d_mats = nullptr;
+        return rc;
+    }
+    if ((rc = e->alloc((void**)&p->d_out, (size_t)cap * 3 * sizeof(double))) < 0) {
+        e->release(p->d_mats, (size_t)cap * mpp * 12 * sizeof(float));
+        e->release(p->d_partials, (size_t)cap * bpp * 2 * sizeof(double));
+        p->d_mats = nullptr;
+        p->d_partials = nullptr;
+        return rc;
+    }
+    HIP_TRY(hipHostMalloc((void**)&p->h_mats, (size_t)cap * mpp * 12 * sizeof(float),
+                          hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&p->h_out, (size_t)cap * 3 * sizeof(double), hipHostMallocDefault));
+    p->cap = cap;
+    return LC2E_OK;
+}
+
+lc2e::FusedArgs make_args(lc2e_pair* p, const float* d_mats, double* d_partials) {
+    lc2e::FusedArgs a{};
+    a.v = p->v;
+    a.s = p->sg;
+    a.src = p->moving->d;
+    a.src_f16 = p->moving->f16;
+    a.stat_img = p->stat_img;
+    a.stat_grad = p->stat_grad;
+    a.mats = d_mats;
+    a.partials = d_partials;
+    a.wmap = nullptr;
+    a.mmap = nullptr;
+    a.r = p->info.patch_radius;
+    a.zchunk = p->zchunk;
+    a.ntx = p->ntx;
+    a.nty = p->nty;
+    a.nzc = p->nzc;
+    a.blocks_per_pose = p->blocks_per_pose;
+    a.mats_per_pose = p->mats_per_pose();
+    return a;
+}
+
+// Enqueue: H->D of the staged matrices, fused kernel, reduction, D->H of the results.
+int enqueue_batch(lc2e_pair* p, int n, hipStream_t st) {
+    const int mpp = p->mats_per_pose();
+    HIP_TRY(hipMemcpyAsync(p->d_mats, p->h_mats, (size_t)n * mpp * 12 * sizeof(float),
+                           hipMemcpyHostToDevice, st));
+    if (p->blocks_per_pose > 0) {
+        lc2e::FusedArgs a = make_args(p, p->d_mats, p->d_partials);
+        const bool timed = p->profile && !p->use_graph;
+        if (timed) HIP_TRY(hipEventRecord(p->ev0, st));
+        HIP_TRY(lc2e::launch_fused(a, p->info.grid_ndim, p->info.src_ndim,
+                                   p->info.high_res_is_moving, n, st));
+        if (timed) HIP_TRY(hipEventRecord(p->ev1, st));
+        HIP_TRY(lc2e::launch_reduce(p->d_partials, p->blocks_per_pose, n, p->d_out, st));
+    } else {
+        // no patch centre fits in the grid: every sum is zero
+        HIP_TRY(lc2e::launch_reduce(p->d_partials, 0, n, p->d_out, st));
+    }
+    HIP_TRY(hipMemcpyAsync(p->h_out, p->d_out, (size_t)n * 3 * sizeof(double),
+                           hipMemcpyDeviceToHost, st));
+    return LC2E_OK;
+}
+
+int run_batch(lc2e_pair* p, int n) {
+    hipStream_t st = p->e->stream;
+    if (p->use_graph) {
+        auto it = p->graphs.find(n);
+        if (it == p->graphs.end()) {
+            GraphEntry g;
+            HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            int rc = enqueue_batch(p, n, st);
+            hipGraph_t graph = nullptr;
+            hipError_t ce = hipStreamEndCapture(st, &graph);
+            if (rc < 0) return rc;
+            if (ce != hipSuccess) return fail(LC2E_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ce));
+            g.graph = graph;
+            HIP_TRY(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+            it = p->graphs.emplace(n, g).first;
+        }
+        HIP_TRY(hipGraphLaunch(it->second.exec, st));
+    } else {
+        int rc = enqueue_batch(p, n, st);
+        if (rc < 0) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    if (p->profile && !p->use_graph && p->blocks_per_pose > 0) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, p->ev0, p->ev1));
+        p->prof_ms += ms;
+        p->prof_launches += 1;
+    }
+    return LC2E_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lc2e_last_error(void) { return g_last_error.c_str(); }
+
+int lc2e_create(int device, size_t hbm_budget_bytes, lc2e_engine** out) {
+    if (!out) return fail(LC2E_ERR_INVALID, "null output handle");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev)
+        return fail(LC2E_ERR_INVALID, "device %d out of range (%d devices)", device, ndev);
+    auto* e = new lc2e_engine();
+    e->device = device;
+    e->budget = hbm_budget_bytes;
+    hipError_t err = hipSetDevice(device);
+    if (err == hipSuccess) err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (err != hipSuccess) {
+        delete e;
+        return fail(LC2E_ERR_HIP, "stream creation failed: %s", hipGetErrorString(err));
+    }
+    *out = e;
+    return LC2E_OK;
+}
+
+void lc2e_destroy(lc2e_engine* e) {
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    (void)hipStreamSynchronize(e->stream);
+    (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+int lc2e_memory(lc2e_engine* e, size_t* used, size_t* budget) {
+    if (!e) return fail(LC2E_ERR_INVALID, "null engine");
+    if (used) *used = e->used;
+    if (budget) *budget = e->budget;
+    return LC2E_OK;
+}
+
+int lc2e_volume_upload(lc2e_engine* e, const void* host, lc2e_dtype dtype, const lc2e_geom* geom,
+                       lc2e_dtype storage, lc2e_volume** out) {
+    if (!e || !host || !geom || !out) return fail(LC2E_ERR_INVALID, "null argument");
+    const int dsz = dtype_size(dtype);
+    if (!dsz) return fail(LC2E_ERR_INVALID, "unknown dtype %d", (int)dtype);
+    for (int a = 0; a < 3; ++a) {
+        if (geom->dims[a] <= 0) return fail(LC2E_ERR_INVALID, "dims must be positive");
+        if (!(geom->spacing[a] > 0.0f)) return fail(LC2E_ERR_INVALID, "spacing must be positive");
+    }
+    const long long n = (long long)geom->dims[0] * geom->dims[1] * geom->dims[2];
+    if (n >= (1LL << 31)) return fail(LC2E_ERR_UNSUPPORTED, "volume exceeds 2^31 voxels");
+    int f16;
+    if (storage == LC2E_STORE_AUTO) f16 = (dtype == LC2E_U8 || dtype == LC2E_F16) ? 1 : 0;
+    else if (storage == LC2E_F16) f16 = 1;
+    else if (storage == LC2E_F32) f16 = 0;
+    else return fail(LC2E_ERR_INVALID, "storage must be F16, F32 or AUTO");
+    auto* v = new lc2e_volume();
+    v->e = e;
+    v->g = *geom;
+    v->f16 = f16;
+    v->bytes = (size_t)n * (f16 ? 2 : 4);
+    int rc = e->alloc(&v->d, v->bytes);
+    if (rc < 0) {
+        delete v;
+        return rc;
+    }
+    // stage the raw bytes on the device, convert there
+    void* staging = nullptr;
+    hipError_t err = hipMalloc(&staging, (size_t)n * dsz);
+    if (err == hipSuccess)
+        err = hipMemcpyAsync(staging, host, (size_t)n * dsz, hipMemcpyHostToDevice, e->stream);
+    if (err == hipSuccess) err = lc2e::launch_convert(staging, (int)dtype, v->d, f16, n, e->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+    if (staging) (void)hipFree(staging);
+    if (err != hipSuccess) {
+        e->release(v->d, v->bytes);
+        delete v;
+        return fail(LC2E_ERR_HIP, "volume upload failed: %s", hipGetErrorString(err));
+    }
+    *out = v;
+    return LC2E_OK;
+}
+
+void lc2e_volume_destroy(lc2e_volume* v) {
+    if (!v) return;
+    (void)hipStreamSynchronize(v->e->stream);
+    v->e->release(v->d, v->bytes);
+    delete v;
+}
+
+int lc2e_pair_create(lc2e_engine* e, lc2e_volume* fixed, lc2e_volume* moving,
+                     const float resampled_spacing[3], int high_res_is_moving,
+                     int center_overlaid, int patch_radius, int dim, lc2e_pair** out) {
+    if (!e || !fixed || !moving || !resampled_spacing || !out)
+        return fail(LC2E_ERR_INVALID, "null argument");
+    if (fixed->e != e || moving->e != e) return fail(LC2E_ERR_INVALID, "volume of another engine");
+    if (dim != 2 && dim != 3) return fail(LC2E_ERR_INVALID, "dim must be 2 or 3");
+    if (patch_radius < 1) return fail(LC2E_ERR_INVALID, "patch radius must be >= 1 (0 selects MI, unsupported)");
+    if (dim == 3 && patch_radius > lc2e::kMaxR3)
+        return fail(LC2E_ERR_UNSUPPORTED, "3D patch radius %d > %d", patch_radius, lc2e::kMaxR3);
+    if (dim == 2 && patch_radius > lc2e::kMaxR2)
+        return fail(LC2E_ERR_UNSUPPORTED, "2D patch radius %d > %d", patch_radius, lc2e::kMaxR2);
+    const int src_ndim = moving->g.dims[2] > 1 ? 3 : 2;
+    if (dim == 3 && (src_ndim != 3 || fixed->g.dims[2] < 1))
+        return fail(LC2E_ERR_INVALID, "3D LC2 needs a 3D moving volume");
+    for (int a = 0; a < dim; ++a)
+        if (!(resampled_spacing[a] > 0.0f)) return fail(LC2E_ERR_INVALID, "resampled spacing must be positive");
+
+    lc2e::PairGeometry pg = lc2e::make_pair_geometry(fixed->g, moving->g, resampled_spacing, dim,
+                                                     src_ndim, center_overlaid != 0);
+    const long long V = (long long)pg.virt.dims[0] * pg.virt.dims[1] * pg.virt.dims[2];
+    if (V >= (1LL << 31)) return fail(LC2E_ERR_UNSUPPORTED, "virtual grid exceeds 2^31 voxels");
+
+    auto* p = new lc2e_pair();
+    p->e = e;
+    p->fixed = fixed;
+    p->moving = moving;
+    p->info.virt = pg.virt;
+    for (int a = 0; a < 3; ++a) {
+        p->info.fixed_center[a] = pg.fixed_center[a];
+        p->info.shift[a] = pg.shift[a];
+    }
+    p->info.grid_ndim = dim;
+    p->info.src_ndim = src_ndim;
+    p->info.high_res_is_moving = high_res_is_moving ? 1 : 0;
+    p->info.patch_radius = patch_radius;
+    p->info.n_frames = dim == 2 ? pg.virt.dims[2] : 1;
+    p->info.per_frame_matrices = 0;
+    const lc2e_geom& vg = pg.virt;
+    p->v = lc2e::VGeom{vg.dims[0], vg.dims[1], vg.dims[2], vg.spacing[0], vg.spacing[1],
+                       vg.spacing[2], vg.origin[0], vg.origin[1], vg.origin[2]};
+    const lc2e_geom& mg = moving->g;
+    p->sg = lc2e::SGeom{mg.dims[0], mg.dims[1], mg.dims[2], mg.spacing[0], mg.spacing[1],
+                        mg.spacing[2], mg.origin[0], mg.origin[1], mg.origin[2]};
+    p->vbytes = (size_t)V * sizeof(float);
+    int rc = e->alloc((void**)&p->stat_img, p->vbytes);
+    if (rc < 0) {
+        delete p;
+        return rc;
+    }
+    const lc2e_geom& fg = fixed->g;
+    lc2e::SGeom fsg{fg.dims[0], fg.dims[1], fg.dims[2], fg.spacing[0], fg.spacing[1],
+                    fg.spacing[2], fg.origin[0], fg.origin[1], fg.origin[2]};
+    hipError_t err = lc2e::launch_warp_static(fixed->d, fixed->f16, fsg, p->v, dim, src_ndim,
+                                              p->stat_img, e->stream);
+    if (err == hipSuccess && !high_res_is_moving) {
+        // the fixed image is the high-resolution one: its |Sobel| is pose-independent
+        rc = e->alloc((void**)&p->stat_grad, p->vbytes);
+        if (rc < 0) {
+            e->release(p->stat_img, p->vbytes);
+            delete p;
+            return rc;
+        }
+        err = lc2e::launch_sobel(p->stat_img, p->v, dim, p->stat_grad, e->stream);
+    }
+    if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+    if (err != hipSuccess) {
+        lc2e_pair_destroy(p);
+        return fail(LC2E_ERR_HIP, "pair set-up failed: %s", hipGetErrorString(err));
+    }
+    update_tiling(p);
+    *out = p;
+    return LC2E_OK;
+}
+
+void lc2e_pair_destroy(lc2e_pair* p) {
+    if (!p) return;
+    (void)hipStreamSynchronize(p->e->stream);
+    free_batch(p);
+    if (p->ev0) (void)hipEventDestroy(p->ev0);
+    if (p->ev1) (void)hipEventDestroy(p->ev1);
+    p->e->release(p->stat_img, p->vbytes);
+    if (p->stat_grad) p->e->release(p->stat_grad, p->vbytes);
+    delete p;
+}
+
+int lc2e_pair_get_info(lc2e_pair* p, lc2e_pair_info* info) {
+    if (!p || !info) return fail(LC2E_ERR_INVALID, "null argument");
+    *info = p->info;
+    return LC2E_OK;
+}
+
+int lc2e_pair_set_per_frame(lc2e_pair* p, int per_frame) {
+    if (!p) return fail(LC2E_ERR_INVALID, "null pair");
+    if (p->info.grid_ndim != 2 && per_frame)
+        return fail(LC2E_ERR_INVALID, "per-frame matrices need a 2D frame stack");
+    free_batch(p);
+    p->info.per_frame_matrices = per_frame ? 1 : 0;
+    return LC2E_OK;
+}
+
+int lc2e_pose_to_matrix(lc2e_pair* p, const double* params, int n, float out12[12]) {
+    if (!p || !params || !out12) return fail(LC2E_ERR_INVALID, "null argument");
+    int rc = lc2e::pose_to_matrix(p->info, params, n, out12);
+    if (rc < 0) return fail(rc, "pose_to_matrix: %d parameters not valid for this pair", n);
+    return LC2E_OK;
+}
+
+int lc2e_eval_batch(lc2e_pair* p, int n, const float* mats, double* lc2, double* sum_measure,
+                    double* sum_w) {
+    if (!p || !mats || n < 0) return fail(LC2E_ERR_INVALID, "invalid argument");
+    if (n == 0) return LC2E_OK;
+    HIP_TRY(hipSetDevice(p->e->device));
+    const int mpp = p->mats_per_pose();
+    const int chunk_max = 65535;
+    int status = LC2E_OK;
+    for (int base = 0; base < n; base += chunk_max) {
+        const int m = (n - base) < chunk_max ? (n - base) : chunk_max;
+        int rc = ensure_capacity(p, m);
+        if (rc < 0) return rc;
+        std::memcpy(p->h_mats, mats + (size_t)base * mpp * 12, (size_t)m * mpp * 12 * sizeof(float));
+        rc = run_batch(p, m);
+        if (rc < 0) return rc;
+        for (int i = 0; i < m; ++i) {
+            const double sm = p->h_out[3 * i], sw = p->h_out[3 * i + 1], l = p->h_out[3 * i + 2];
+            if (lc2) lc2[base + i] = l;
+            if (sum_measure) sum_measure[base + i] = sm;
+            if (sum_w) sum_w[base + i] = sw;
+            if (!(sw > 0.0) || !std::isfinite(sm / sw)) status = LC2E_OK_DEGENERATE;
+        }
+    }
+    return status;
+}
+
+int lc2e_eval(lc2e_pair* p, const float* m12, double* lc2) {
+    return lc2e_eval_batch(p, 1, m12, lc2, nullptr, nullptr);
+}
+
+int lc2e_eval_batch_async(lc2e_pair* p, int n, const float* d_mats, double* d_out, void* stream) {
+    if (!p || !d_mats || !d_out || n <= 0) return fail(LC2E_ERR_INVALID, "invalid argument");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : p->e->stream;
+    // partials are per pair; size them for n
+    int rc = ensure_capacity(p, n);
+    if (rc < 0) return rc;
+    if (p->blocks_per_pose > 0) {
+        lc2e::FusedArgs a = make_args(p, d_mats, p->d_partials);
+        HIP_TRY(lc2e::launch_fused(a, p->info.grid_ndim, p->info.src_ndim,
+                                   p->info.high_res_is_moving, n, st));
+    }
+    HIP_TRY(lc2e::launch_reduce(p->d_partials, p->blocks_per_pose, n, d_out, st));
+    return LC2E_OK;
+}
+
+int lc2e_sync(lc2e_engine* e) {
+    if (!e) return fail(LC2E_ERR_INVALID, "null engine");
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipDeviceSynchronize());
+    return LC2E_OK;
+}
+
+int lc2e_debug_fetch(lc2e_pair* p, lc2e_stage stage, const float* m12, void* host) {
+    if (!p || !host || !m12) return fail(LC2E_ERR_INVALID, "null argument");
+    hipStream_t st = p->e->stream;
+    const long long V = (long long)p->v.X * p->v.Y * p->v.Z;
+    const int mpp = p->mats_per_pose();
+    int rc = ensure_capacity(p, 1);
+    if (rc < 0) return rc;
+    HIP_TRY(hipMemcpy(p->d_mats, m12, (size_t)mpp * 12 * sizeof(float), hipMemcpyHostToDevice));
+    lc2e::FusedArgs a = make_args(p, p->d_mats, p->d_partials);
+    const bool movi = p->info.high_res_is_moving != 0;
+    if (stage == LC2E_STAGE_WARP_I || stage == LC2E_STAGE_WARP_US || stage == LC2E_STAGE_GRAD) {
+        const bool want_moving = (stage == LC2E_STAGE_WARP_I) == movi || (stage == LC2E_STAGE_GRAD && movi);
+        float* buf = nullptr;
+        float* buf2 = nullptr;
+        HIP_TRY(hipMalloc(&buf, V * sizeof(float)));
+        hipError_t err = hipSuccess;
+        const float* src = p->stat_img;
+        if (stage == LC2E_STAGE_GRAD && !movi) {
+            src = p->stat_grad;
+        } else if (want_moving) {
+            err = lc2e::launch_warp_moving(a, p->info.grid_ndim, p->info.src_ndim, buf, st);
+            src = buf;
+            if (err == hipSuccess && stage == LC2E_STAGE_GRAD) {
+                err = hipMalloc(&buf2, V * sizeof(float));
+                if (err == hipSuccess) err = lc2e::launch_sobel(buf, p->v, p->info.grid_ndim, buf2, st);
+                src = buf2;
+            }
+        }
+        if (err == hipSuccess) err = hipMemcpyAsync(host, src, V * sizeof(float), hipMemcpyDeviceToHost, st);
+        if (err == hipSuccess) err = hipStreamSynchronize(st);
+        (void)hipFree(buf);
+        if (buf2) (void)hipFree(buf2);
+        if (err != hipSuccess) return fail(LC2E_ERR_HIP, "debug fetch failed: %s", hipGetErrorString(err));
+        return LC2E_OK;
+    }
+    if (stage != LC2E_STAGE_W && stage != LC2E_STAGE_MEASURE)
+        return fail(LC2E_ERR_INVALID, "unknown stage %d", (int)stage);
+    double* maps = nullptr;
+    HIP_TRY(hipMalloc(&maps, 2 * V * sizeof(double)));
+    hipError_t err = hipMemsetAsync(maps, 0, 2 * V * sizeof(double), st);
+    a.wmap = maps;
+    a.mmap = maps + V;
+    if (err == hipSuccess && p->blocks_per_pose > 0)
+        err = lc2e::launch_fused(a, p->info.grid_ndim, p->info.src_ndim, p->info.high_res_is_moving, 1, st);
+    if (err == hipSuccess)
+        err = hipMemcpyAsync(host, stage == LC2E_STAGE_W ? a.wmap : a.mmap, V * sizeof(double),
+                             hipMemcpyDeviceToHost, st);
+    if (err == hipSuccess) err = hipStreamSynchronize(st);
+    (void)hipFree(maps);
+    if (err != hipSuccess) return fail(LC2E_ERR_HIP, "debug fetch failed: %s", hipGetErrorString(err));
+    return LC2E_OK;
+}
+
+int lc2e_pair_set_tuning(lc2e_pair* p, int zchunk) {
+    if (!p || zchunk < 0) return fail(LC2E_ERR_INVALID, "invalid argument");
+    (void)hipStreamSynchronize(p->e->stream);
+    free_batch(p);
+    p->zchunk = zchunk;
+    return update_tiling(p);
+}
+
+int lc2e_pair_get_tuning(lc2e_pair* p, int* zchunk) {
+    if (!p || !zchunk) return fail(LC2E_ERR_INVALID, "invalid argument");
+    *zchunk = p->zchunk;
+    return LC2E_OK;
+}
+
+int lc2e_pair_profile(lc2e_pair* p, int enable) {
+    if (!p) return fail(LC2E_ERR_INVALID, "null pair");
+    if (enable && !p->ev0) {
+        HIP_TRY(hipEventCreate(&p->ev0));
+        HIP_TRY(hipEventCreate(&p->ev1));
+    }
+    p->profile = enable != 0;
+    p->prof_ms = 0.0;
+    p->prof_launches = 0;
+    return LC2E_OK;
+}
+
+int lc2e_pair_profile_read(lc2e_pair* p, double* kernel_ms, long long* launches) {
+    if (!p) return fail(LC2E_ERR_INVALID, "null pair");
+    if (kernel_ms) *kernel_ms = p->prof_ms;
+    if (launches) *launches = p->prof_launches;
+    return LC2E_OK;
+}
+
+int lc2e_pair_use_graph(lc2e_pair* p, int enable) {
+    if (!p) return fail(LC2E_ERR_INVALID, "null pair");
+    if (!enable) drop_graphs(p);
+    p->use_graph = enable != 0;
+    return LC2E_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,63 @@
+// Internal interface between the engine (lc2e_engine.cpp) and the device code
+// (lc2e_kernels.hip). Not part of the public C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace lc2e {
+
+constexpr int kTile = 16;      // output tile edge (x and y) of the fused kernels
+constexpr int kThreads = 256;  // work-group size of the fused kernels (16 x 16 outputs)
+constexpr int kMaxR3 = 4;      // largest patch radius of the 3D fused kernel
+constexpr int kMaxR2 = 10;     // largest patch radius of the 2D fused kernel
+
+struct VGeom {  // virtual grid
+    int X, Y, Z;
+    float sx, sy, sz, ox, oy, oz;
+};
+
+struct SGeom {  // a resident image
+    int X, Y, Z;
+    float sx, sy, sz, ox, oy, oz;
+};
+
+struct FusedArgs {
+    VGeom v;
+    SGeom s;                 // moving image
+    const void* src;         // moving image data (float or half)
+    const float* stat_img;   // fixed image resampled on the virtual grid
+    const float* stat_grad;  // |Sobel| of stat_img when it is the high-resolution image
+    const float* mats;       // [pose][mats_per_pose][12] VirtualToMoving, 3x4 row-major
+    double* partials;        // [pose][blocks_per_pose][2] (sum measure, sum w)
+    double* wmap;            // optional per-voxel w (single pose), may be null
+    double* mmap;            // optional per-voxel measure, may be null
+    int r;
+    int zchunk;              // 3D: output planes per work-group
+    int ntx, nty, nzc;       // tiles along x, y and z chunks (3D) / frames (2D)
+    int blocks_per_pose;
+    int mats_per_pose;
+    int src_f16;             // moving image stored as half
+};
+
+// Number of work-groups one pose needs (0 if the grid has no valid patch centre).
+int fused_blocks_3d(const VGeom& v, int r, int zchunk, int* ntx, int* nty, int* nzc);
+int fused_blocks_2d(const VGeom& v, int r, int* ntx, int* nty);
+
+hipError_t launch_fused(const FusedArgs& a, int grid_ndim, int src_ndim, int high_res_is_moving,
+                        int n_poses, hipStream_t st);
+hipError_t launch_reduce(const double* partials, int blocks_per_pose, int n_poses, double* out,
+                         hipStream_t st);
+// Fixed image -> virtual grid at identity (frames of a 2D stack one by one).
+hipError_t launch_warp_static(const void* fixed, int fixed_f16, const SGeom& fg, const VGeom& v,
+                              int grid_ndim, int src_ndim, float* out, hipStream_t st);
+// Moving image -> virtual grid at one pose (debug stages).
+hipError_t launch_warp_moving(const FusedArgs& a, int grid_ndim, int src_ndim, float* out,
+                              hipStream_t st);
+// |Sobel| of a virtual-grid image (3D, or per frame for 2D grids), separable order.
+hipError_t launch_sobel(const float* img, const VGeom& v, int grid_ndim, float* out,
+                        hipStream_t st);
+// Host bytes -> resident storage (u8/i16/f16/f32 -> f16/f32).
+hipError_t launch_convert(const void* d_in, int in_dtype, void* d_out, int out_f16, long long n,
+                          hipStream_t st);
+
+}  // namespace lc2e

@@ -1,0 +1,274 @@
+"""GPU parity: the HIP engine (through the C ABI) against the oracle on the same inputs.
+
+Bars: FP32 stages (warps, |Sobel|) bit-exact; per-voxel w / measure within 1e-9
+relative (FP64 moment order differs); LC2 within 1e-9 absolute here, against the
+1e-5 the workload requires.
+"""
+import numpy as np
+import pytest
+
+from lc2registration_amd import engine as E
+from lc2registration_amd import geometry as geo
+from lc2registration_amd import synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+LC2_TOL = 1e-9
+
+
+def _ogeom(img):
+    return O.geom(img.dims, img.spacing, img.origin)
+
+
+def _pair_for(engine, w, storage=E.STORE_AUTO, moving_data=None, fixed_data=None, r=None):
+    f, m = w.fixed, w.moving
+    fv = engine.upload(f.data if fixed_data is None else fixed_data, f.spacing, f.origin)
+    mv = engine.upload(m.data if moving_data is None else moving_data, m.spacing, m.origin,
+                       storage)
+    return engine.pair(fv, mv, w.resampled_spacing, w.high_res_is_moving, w.center_overlaid,
+                       r or w.patch_radius, w.ndim)
+
+
+def _oracle_eval(w, M, r=None, moving_data=None, src_ndim=None, grid_ndim=None, maps=False):
+    f, m = w.fixed, w.moving
+    fg, mg = _ogeom(f), _ogeom(m)
+    nd = len(f.dims)
+    vd = O.virtual_dims(fg, w.resampled_spacing, nd)
+    vg = O.geom(vd, w.resampled_spacing, f.origin)
+    mov = (m.data if moving_data is None else moving_data).astype(np.float32)
+    fix = f.data.astype(np.float32)
+    src_ndim = src_ndim or len(m.dims)
+    grid_ndim = grid_ndim or w.ndim
+    if src_ndim == 3:
+        Wf = O.warp_3d(fix, fg, vg, [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0])
+        Wm = O.warp_3d(mov, mg, vg, np.asarray(M).ravel()[:12])
+    else:
+        Wf = O.warp_2d(fix, fg, vg, [1, 0, 0, 0, 1, 0])
+        M3 = np.asarray(M).reshape(3, 4)
+        Wm = O.warp_2d(mov, mg, vg, [M3[0, 0], M3[0, 1], M3[0, 3], M3[1, 0], M3[1, 1], M3[1, 3]])
+    I, U = (Wm, Wf) if w.high_res_is_moving else (Wf, Wm)
+    G = O.sobel(I, "sep")
+    res = O.lc2_exact(U, I, G, r or w.patch_radius, maps=maps)
+    return (res, I, U, G) if not maps else (res[0], I, U, G, res[1], res[2])
+
+
+@pytest.fixture(scope="module")
+def w3():
+    return synth.small3d()
+
+
+def test_kidney_2d_stages_and_lc2(engine, kidney):
+    """Reference data: static CT warp bit-exact vs the reference's dump; US warp at the
+    recovered pose bit-exact vs the oracle and within 1e-3 of the reference's dump;
+    LC2 and per-voxel maps vs the oracle for the reference's 2D patch radii 3 and 9."""
+    man = kidney["manifest"]
+    ct = kidney["ct"]
+    us = kidney["us"]
+    fv = engine.upload(ct, man["fixed"]["spacing"], man["fixed"]["origin"])
+    mv = engine.upload(us, man["moving"]["spacing"], man["moving"]["origin"])
+    rec = man["rigid_dumps"]["moving.bin"]
+    for r in (3, 9):
+        p = engine.pair(fv, mv, (1.0, 1.0), False, True, r, 2)
+        assert p.virt_dims[:2] == (164, 151)
+        M = p.pose_to_matrix(rec["pose_tx_ty_rdeg"])
+        wi = p.debug_fetch(E.STAGE_WARP_I, M)[0]
+        assert np.array_equal(wi, kidney["fixed.bin"])
+        wu = p.debug_fetch(E.STAGE_WARP_US, M)[0]
+        vg = O.geom((164, 151), (1.0, 1.0), (0, 0))
+        ow = O.warp_2d(us, O.geom(man["moving"]["dims"], man["moving"]["spacing"], (0, 0)), vg,
+                       [M[0], M[1], M[3], M[4], M[5], M[7]])
+        assert np.array_equal(wu, ow)
+        assert np.abs(wu - kidney["moving.bin"]).max() <= 1.5e-3
+        g = p.debug_fetch(E.STAGE_GRAD, M)[0]
+        assert np.array_equal(g, O.sobel(wi, "sep"))
+        res, wm, mm = O.lc2_exact(wu, wi, g, r, maps=True)
+        l = p.eval(M)
+        assert abs(l - res.lc2) < LC2_TOL, (r, l, res.lc2)
+        gw = p.debug_fetch(E.STAGE_W, M)[0]
+        gm = p.debug_fetch(E.STAGE_MEASURE, M)[0]
+        assert np.array_equal(gw > 0, wm > 0)
+        assert np.allclose(gw, wm, rtol=1e-9, atol=1e-12)
+        assert np.allclose(gm, mm, rtol=1e-7, atol=1e-9)
+        p.close()
+    fv.close()
+    mv.close()
+
+
+@pytest.mark.parametrize("r", [1, 2, 3, 4])
+def test_3d_moving_highres_parity(engine, w3, r):
+    p = _pair_for(engine, w3, r=r)
+    rng = np.random.default_rng(r)
+    poses = [w3.true_pose] + [synth.perturb(w3.true_pose, rng, 3) for _ in range(2)]
+    for pose in poses:
+        M = p.pose_to_matrix(pose)
+        res, I, U, G = _oracle_eval(w3, M, r=r)
+        assert np.array_equal(p.debug_fetch(E.STAGE_WARP_I, M), I)
+        assert np.array_equal(p.debug_fetch(E.STAGE_WARP_US, M), U)
+        assert np.array_equal(p.debug_fetch(E.STAGE_GRAD, M), G)
+        l = p.eval(M)
+        assert res.n_valid > 1000
+        assert abs(l - res.lc2) < LC2_TOL, (r, l, res.lc2)
+    p.close()
+
+
+def test_3d_maps_match_oracle(engine, w3):
+    p = _pair_for(engine, w3)
+    M = p.pose_to_matrix(w3.true_pose)
+    res, I, U, G, wm, mm = _oracle_eval(w3, M, maps=True)
+    gw = p.debug_fetch(E.STAGE_W, M)
+    gm = p.debug_fetch(E.STAGE_MEASURE, M)
+    assert np.array_equal(gw > 0, wm > 0)
+    assert np.allclose(gw, wm, rtol=1e-9, atol=1e-12)
+    assert np.allclose(gm, mm, rtol=1e-7, atol=1e-9)
+    p.close()
+
+
+def test_3d_fixed_highres_parity(engine, w3):
+    """HighResolutionModality: Fixed in 3D -- the fixed MR is I (static |Sobel|), the
+    moving US is warped per pose."""
+    f, m = w3.moving, w3.fixed  # roles swapped
+    w = synth.Workload("swap", f, m, (1.0, 1.0, 1.0), 2, False, True, 3, w3.true_pose)
+    p = _pair_for(engine, w)
+    rng = np.random.default_rng(9)
+    for pose in [np.r_[0, 0, 0, 0, 0, 0, 1, 1, 1, 0, 0, 0], synth.perturb(np.r_[0, 0, 0, 0, 0, 0, 1, 1, 1, 0, 0, 0], rng, 3)]:
+        M = p.pose_to_matrix(pose)
+        res, I, U, G = _oracle_eval(w, M)
+        assert np.array_equal(p.debug_fetch(E.STAGE_WARP_US, M), U)
+        assert np.array_equal(p.debug_fetch(E.STAGE_GRAD, M), G)
+        l = p.eval(M)
+        assert res.n_valid > 100
+        assert abs(l - res.lc2) < LC2_TOL
+    p.close()
+
+
+def test_2d_affine_cfg0_parity(engine):
+    w = synth.config0(n=96, patch_radius=3)
+    p = _pair_for(engine, w)
+    rng = np.random.default_rng(1)
+    for pose in [w.true_pose, synth.perturb(w.true_pose, rng, 2)]:
+        M = p.pose_to_matrix(pose)
+        assert np.allclose(M.reshape(3, 4), geo.embed_2d(geo.pose_to_matrix_2d(
+            pose, w.fixed.dims, w.fixed.spacing, w.fixed.origin, w.moving.dims,
+            w.moving.spacing, w.moving.origin, w.resampled_spacing)), atol=1e-5)
+        res, I, U, G = _oracle_eval(w, M)
+        assert np.array_equal(p.debug_fetch(E.STAGE_WARP_US, M)[0], U)
+        l = p.eval(M)
+        assert abs(l - res.lc2) < LC2_TOL
+    p.close()
+
+
+def test_2d_to_3d_slice_parity(engine, w3):
+    """2D->3D (Interpolator2DTo3D): a one-slice fixed US, the 3D MR moving, 2D LC2."""
+    us = w3.fixed
+    sl = us.data[us.data.shape[0] // 2][None].copy()
+    fixed = synth.Image(sl, (us.dims[0], us.dims[1], 1), us.spacing,
+                        (us.origin[0], us.origin[1], us.origin[2] + us.spacing[2] * (us.dims[2] // 2)))
+    w = synth.Workload("2dto3d", fixed, w3.moving, us.spacing, 3, True, False, 2, w3.true_pose)
+    p = _pair_for(engine, w)
+    assert p.info.grid_ndim == 2 and p.info.src_ndim == 3
+    M = p.pose_to_matrix(w3.true_pose)
+    res, I, U, G = _oracle_eval(w, M, src_ndim=3, grid_ndim=2)
+    # the oracle's 2D LC2 works on (Y, X) planes
+    res2 = O.lc2_exact(U[0], I[0], O.sobel(I[0], "sep"), 3)
+    assert np.array_equal(p.debug_fetch(E.STAGE_WARP_I, M), I)
+    l = p.eval(M)
+    assert res2.n_valid > 100
+    assert abs(l - res2.lc2) < LC2_TOL
+    p.close()
+
+
+def test_frame_stack_per_frame_matrices(engine, w3):
+    """A stack of tracked frames with one matrix per frame: sums equal the per-frame sums."""
+    us = w3.fixed
+    zs = [8, 14, 20]
+    stack = np.ascontiguousarray(us.data[zs])
+    fv = engine.upload(stack, us.spacing, us.origin)
+    mv = engine.upload(w3.moving.data, w3.moving.spacing, w3.moving.origin)
+    p = engine.pair(fv, mv, us.spacing, True, False, 2, 2)
+    p.set_per_frame(True)
+    base = p.pose_to_matrix(w3.true_pose).reshape(3, 4).astype(np.float64)
+    mats = []
+    for z in zs:  # frame z lives at height origin_z + z*spacing_z: fold into its matrix
+        T = np.eye(4)
+        T[2, 3] = z * us.spacing[2]
+        mats.append((np.vstack([base, [0, 0, 0, 1]]) @ T)[:3].astype(np.float32))
+    mats = np.stack(mats)
+    br = p.eval_batch(mats[None])
+    sm = sw = 0.0
+    for k, z in enumerate(zs):
+        fk = engine.upload(np.ascontiguousarray(us.data[z][None]), us.spacing, us.origin)
+        pk = engine.pair(fk, mv, us.spacing, True, False, 2, 2)
+        r = pk.eval_batch(mats[k][None])
+        sm += r.sum_measure[0]
+        sw += r.sum_w[0]
+        pk.close()
+        fk.close()
+    assert abs(br.sum_measure[0] - sm) < 1e-9 * max(1.0, abs(sm))
+    assert abs(br.sum_w[0] - sw) < 1e-9 * max(1.0, abs(sw))
+    p.close()
+
+
+def test_batch_equals_single_and_deterministic(engine, w3):
+    p = _pair_for(engine, w3)
+    rng = np.random.default_rng(5)
+    mats = np.stack([p.pose_to_matrix(synth.perturb(w3.true_pose, rng, 3)) for _ in range(7)])
+    b = p.eval_batch(mats)
+    singles = np.array([p.eval(m) for m in mats])
+    assert np.array_equal(b.lc2, singles)
+    b2 = p.eval_batch(mats)
+    assert np.array_equal(b.lc2, b2.lc2)
+    p.close()
+
+
+def test_fp16_storage_matches_oracle_on_rounded_input(engine, w3):
+    m16 = w3.moving.data.astype(np.float16)
+    p = _pair_for(engine, w3, moving_data=m16)
+    M = p.pose_to_matrix(w3.true_pose)
+    res, I, U, G = _oracle_eval(w3, M, moving_data=m16.astype(np.float32))
+    assert abs(p.eval(M) - res.lc2) < LC2_TOL
+    p.close()
+
+
+def test_degenerate_constant_us(engine, w3):
+    w = synth.Workload("const", synth.Image(np.full_like(w3.fixed.data, 7), w3.fixed.dims,
+                                             w3.fixed.spacing, w3.fixed.origin), w3.moving,
+                       w3.resampled_spacing, 2, True, False, 3, w3.true_pose)
+    p = _pair_for(engine, w)
+    M = p.pose_to_matrix(w3.true_pose)
+    b = p.eval_batch(M[None])
+    assert b.status == E.OK_DEGENERATE and b.lc2[0] == 0.0 and b.sum_w[0] == 0.0
+    p.close()
+
+
+def test_budget_rejects_oversized_pair(w3):
+    e = E.Engine(0, hbm_budget=200_000)
+    with pytest.raises(E.LC2Error) as ei:
+        e.upload(w3.moving.data, w3.moving.spacing, w3.moving.origin)
+    assert ei.value.code == -3
+    e.close()
+
+
+def test_graph_replay_matches_eager(engine, w3):
+    p = _pair_for(engine, w3)
+    rng = np.random.default_rng(8)
+    mats = np.stack([p.pose_to_matrix(synth.perturb(w3.true_pose, rng, 3)) for _ in range(4)])
+    eager = p.eval_batch(mats).lc2
+    p.use_graph(True)
+    for k in range(3):
+        g = p.eval_batch(mats).lc2
+        assert np.array_equal(g, eager), k
+        assert p.eval(mats[k]) == eager[k]  # other work between replays
+    p.use_graph(False)
+    p.close()
+
+
+@pytest.mark.parametrize("zchunk", [4, 9, 32])
+def test_zchunk_does_not_change_result_beyond_rounding(engine, w3, zchunk):
+    p = _pair_for(engine, w3)
+    M = p.pose_to_matrix(w3.true_pose)
+    ref = _oracle_eval(w3, M)[0].lc2
+    p.set_tuning(zchunk)
+    assert p.get_tuning() == zchunk
+    assert abs(p.eval(M) - ref) < LC2_TOL
+    p.close()
