@@ -707,12 +707,13 @@ hipError_t launch3(const FusedArgs& a, bool movi, dim3 grid, hipStream_t st) {
 template <bool MOVI, bool SRC3D>
 hipError_t launch2(const FusedArgs& a, dim3 grid, hipStream_t st) {
     const size_t lds = lds_bytes_2d(a.r);
-    static bool configured[kMaxR2 + 1] = {false};
-    if (!configured[a.r]) {
+    static bool configured = false;  // allow the largest radius once; smaller ones fit under it
+    if (!configured) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused2d<MOVI, SRC3D>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds_bytes_2d(kMaxR2));
         if (e != hipSuccess) return e;
-        configured[a.r] = true;
+        configured = true;
     }
     hipLaunchKernelGGL((k_fused2d<MOVI, SRC3D>), grid, dim3(kThreads), lds, st, a);
     return hipGetLastError();
