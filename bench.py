@@ -81,7 +81,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--poses", type=int, default=32, help="candidate poses per fused launch")
-    ap.add_argument("--zchunk", type=int, default=0)
+    ap.add_argument("--zchunk", type=int, default=0, help="0: autotune (cached)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds per CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip single-pose / fp16 figures")
@@ -102,13 +102,17 @@ def main():
     fv = eng.upload(f.data, f.spacing, f.origin)
     mv = eng.upload(m.data, m.spacing, m.origin, E.F32)
     pair = eng.pair(fv, mv, w.resampled_spacing, True, False, w.patch_radius, 3)
-    if args.zchunk:
-        pair.set_tuning(args.zchunk)
     V = int(np.prod(pair.virt_dims))
     rng = np.random.default_rng(123)
     poses = [w.true_pose] + [synth.perturb(w.true_pose, rng, 3) for _ in range(args.poses - 1)]
     mats = np.stack([pair.pose_to_matrix(p) for p in poses])
     P = len(mats)
+    if args.zchunk:
+        pair.set_tuning(args.zchunk)
+    else:
+        from lc2registration_amd import autotune
+        zc = autotune.tune(pair, mats)
+        log(f"autotuned z-chunk {zc}")
 
     for _ in range(args.warmup):
         pair.eval_batch(mats)

@@ -140,6 +140,11 @@ int lc2e_debug_fetch(lc2e_pair* p, lc2e_stage stage, const float* m12, void* hos
 /* Launch configuration of the fused kernel (z planes per work-group; 0 = heuristic). */
 int lc2e_pair_set_tuning(lc2e_pair* p, int zchunk);
 int lc2e_pair_get_tuning(lc2e_pair* p, int* zchunk);
+/* Autotuner: times the fused kernel over candidate z-chunks (HIP events, `iters` launches
+ * each after one warm-up) for batches of n poses `mats`, keeps the fastest and returns it.
+ * 2D pairs have nothing to tune and return 1. No counterpart in the reference. */
+int lc2e_pair_autotune(lc2e_pair* p, int n, const float* mats, int iters, int* best_zchunk,
+                       double* best_ms);
 
 /* Kernel timing with HIP events recorded around every fused-kernel launch on the engine's
  * stream (eager path only). enable = 1 resets the counters. read returns the summed

@@ -1,0 +1,48 @@
+/*
+ * lc2reg.h -- C ABI of the registration layer above the LC2 engine: a whole YAML-driven
+ * registration, and the pieces it is built from (optimizers, configuration I/O) for
+ * callers and tests that reach them from C or through an FFI.
+ * Statuses: 0 ok, negative on error (message in lc2reg_last_error()).
+ */
+#ifndef LC2REG_H
+#define LC2REG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char* lc2reg_last_error(void);
+
+/* One registration as the reference's CLI runs it for -m 2D | 3D | 2DTo3D -c <yaml>
+ * (main.cu:728-827): Interpolator + Optimizers from the YAML, DIRECT stage and BOBYQA
+ * rounds, GetOptimalTransform, ExportResults (writes <yaml stem>_result.<ext>), WriteOut
+ * when `write_images` is set. Outputs (any may be NULL): the optimum parameter vector
+ * (12 values for 3D / 2DTo3D, 6 for 2D), the last LC2, the evaluation count, the
+ * moving-to-fixed matrix (16 or 9 floats, row-major). */
+int lc2reg_register_yaml(const char* mode, const char* yaml_path, int write_images,
+                         double* out_params, double* out_lc2, int* out_nevals, float* out_matrix);
+
+/* LC2 of one parameter vector for a YAML-configured pair (Interpolator*::GenerateMappers +
+ * Interpolate + GetSimilarityMeasure with the YAML's PatchSize). */
+int lc2reg_evaluate_yaml(const char* mode, const char* yaml_path, const double* params, int n_params,
+                         double* out_lc2);
+
+/* Batch objective: F[i] = f(X[i*d .. i*d+d-1]) for i < n. */
+typedef void (*lc2reg_batch_fn)(int n, int d, const double* X, double* F, void* user);
+
+/* The optimizers behind Optimizers* (see optim.hpp): DIRECT-L and the bound-constrained
+ * quadratic-model trust region used for the BOBYQA rounds. */
+int lc2reg_minimize_direct(int d, const double* lb, const double* ub, int maxeval,
+                           lc2reg_batch_fn f, void* user, double* x_out, double* f_out,
+                           int* nevals);
+int lc2reg_minimize_qmtr(int d, const double* x0, const double* lb, const double* ub,
+                         double xtol_rel, int maxeval, lc2reg_batch_fn f, void* user,
+                         double* x_out, double* f_out, int* nevals);
+
+/* Parse a parameters*.yaml with the configuration reader and write it back out. */
+int lc2reg_yaml_roundtrip(const char* in_path, const char* out_path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

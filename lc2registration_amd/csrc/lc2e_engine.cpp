@@ -93,6 +93,7 @@ struct lc2e_pair {
     lc2e::SGeom sg{};
     float* stat_img = nullptr;
     float* stat_grad = nullptr;
+    double* ustat = nullptr;  // [2][V]: patch mean and centred energy of a static ultrasound
     size_t vbytes = 0;
     int zchunk = 0;
     int ntx = 0, nty = 0, nzc = 0, blocks_per_pose = 0;
@@ -133,8 +134,8 @@ int heuristic_zchunk(const lc2e_pair* p) {
     int ntx, nty, nzc;
     const int cols = (lc2e::fused_blocks_3d(p->v, r, 1 << 20, &ntx, &nty, &nzc) > 0) ? ntx * nty : 1;
     const int ez = p->v.Z - 2 * r;
-    int zc = 32;
-    while (zc > 8 && cols * ((ez + zc - 1) / zc) < 1024) zc /= 2;
+    int zc = 64;
+    while (zc > 8 && cols * ((ez + zc - 1) / zc) < 256) zc /= 2;
     return zc;
 }
 
@@ -206,6 +207,9 @@ lc2e::FusedArgs make_args(lc2e_pair* p, const float* d_mats, double* d_partials)
     a.src_f16 = p->moving->f16;
     a.stat_img = p->stat_img;
     a.stat_grad = p->stat_grad;
+    const size_t V = p->vbytes / sizeof(float);
+    a.ustat_mean = p->ustat;
+    a.ustat_cyy = p->ustat ? p->ustat + V : nullptr;
     a.mats = d_mats;
     a.partials = d_partials;
     a.wmap = nullptr;
@@ -333,11 +337,21 @@ int lc2e_volume_upload(lc2e_engine* e, const void* host, lc2e_dtype dtype, const
     v->e = e;
     v->g = *geom;
     v->f16 = f16;
-    v->bytes = (size_t)n * (f16 ? 2 : 4);
+    v->bytes = (size_t)n * (f16 ? 2 : 4) + lc2e::kVolumePad;
     int rc = e->alloc(&v->d, v->bytes);
     if (rc < 0) {
         delete v;
         return rc;
+    }
+    {
+        // zeroed tail: the samplers read voxel pairs (x, x + 1) in one load
+        hipError_t pe = hipMemsetAsync(static_cast<char*>(v->d) + (v->bytes - lc2e::kVolumePad), 0,
+                                       lc2e::kVolumePad, e->stream);
+        if (pe != hipSuccess) {
+            e->release(v->d, v->bytes);
+            delete v;
+            return fail(LC2E_ERR_HIP, "volume padding failed: %s", hipGetErrorString(pe));
+        }
     }
     // stage the raw bytes on the device, convert there
     void* staging = nullptr;
@@ -405,8 +419,7 @@ int lc2e_pair_create(lc2e_engine* e, lc2e_volume* fixed, lc2e_volume* moving,
     p->v = lc2e::VGeom{vg.dims[0], vg.dims[1], vg.dims[2], vg.spacing[0], vg.spacing[1],
                        vg.spacing[2], vg.origin[0], vg.origin[1], vg.origin[2]};
     const lc2e_geom& mg = moving->g;
-    p->sg = lc2e::SGeom{mg.dims[0], mg.dims[1], mg.dims[2], mg.spacing[0], mg.spacing[1],
-                        mg.spacing[2], mg.origin[0], mg.origin[1], mg.origin[2]};
+    p->sg = lc2e::make_sgeom(mg.dims, mg.spacing, mg.origin);
     p->vbytes = (size_t)V * sizeof(float);
     int rc = e->alloc((void**)&p->stat_img, p->vbytes);
     if (rc < 0) {
@@ -414,10 +427,20 @@ int lc2e_pair_create(lc2e_engine* e, lc2e_volume* fixed, lc2e_volume* moving,
         return rc;
     }
     const lc2e_geom& fg = fixed->g;
-    lc2e::SGeom fsg{fg.dims[0], fg.dims[1], fg.dims[2], fg.spacing[0], fg.spacing[1],
-                    fg.spacing[2], fg.origin[0], fg.origin[1], fg.origin[2]};
+    lc2e::SGeom fsg = lc2e::make_sgeom(fg.dims, fg.spacing, fg.origin);
     hipError_t err = lc2e::launch_warp_static(fixed->d, fixed->f16, fsg, p->v, dim, src_ndim,
                                               p->stat_img, e->stream);
+    if (err == hipSuccess && high_res_is_moving) {
+        // the fixed image is the ultrasound: its patch statistics are pose-independent
+        rc = e->alloc((void**)&p->ustat, 2 * (size_t)V * sizeof(double));
+        if (rc < 0) {
+            e->release(p->stat_img, p->vbytes);
+            delete p;
+            return rc;
+        }
+        err = lc2e::launch_ustats(p->stat_img, p->v, dim, patch_radius, p->ustat, p->ustat + V,
+                                  e->stream);
+    }
     if (err == hipSuccess && !high_res_is_moving) {
         // the fixed image is the high-resolution one: its |Sobel| is pose-independent
         rc = e->alloc((void**)&p->stat_grad, p->vbytes);
@@ -446,6 +469,7 @@ void lc2e_pair_destroy(lc2e_pair* p) {
     if (p->ev1) (void)hipEventDestroy(p->ev1);
     p->e->release(p->stat_img, p->vbytes);
     if (p->stat_grad) p->e->release(p->stat_grad, p->vbytes);
+    if (p->ustat) p->e->release(p->ustat, 2 * (p->vbytes / sizeof(float)) * sizeof(double));
     delete p;
 }
 
@@ -582,6 +606,56 @@ int lc2e_pair_set_tuning(lc2e_pair* p, int zchunk) {
     free_batch(p);
     p->zchunk = zchunk;
     return update_tiling(p);
+}
+
+int lc2e_pair_autotune(lc2e_pair* p, int n, const float* mats, int iters, int* best_zchunk,
+                       double* best_ms) {
+    if (!p || !mats || n <= 0 || iters <= 0) return fail(LC2E_ERR_INVALID, "invalid argument");
+    if (p->info.grid_ndim != 3) {
+        if (best_zchunk) *best_zchunk = 1;
+        if (best_ms) *best_ms = 0.0;
+        return LC2E_OK;
+    }
+    const int ez = p->v.Z - 2 * p->info.patch_radius;
+    static const int cand[] = {8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256};
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int best = 0;
+    double bms = 1e300;
+    hipStream_t st = p->e->stream;
+    const bool graph = p->use_graph;
+    p->use_graph = false;
+    for (int zc : cand) {
+        if (zc > ez && zc != cand[0]) break;
+        int rc = lc2e_pair_set_tuning(p, zc);
+        if (rc < 0) return rc;
+        rc = ensure_capacity(p, n);
+        if (rc < 0) return rc;
+        HIP_TRY(hipMemcpy(p->d_mats, mats, (size_t)n * p->mats_per_pose() * 12 * sizeof(float),
+                          hipMemcpyHostToDevice));
+        lc2e::FusedArgs a = make_args(p, p->d_mats, p->d_partials);
+        HIP_TRY(lc2e::launch_fused(a, 3, p->info.src_ndim, p->info.high_res_is_moving, n, st));
+        HIP_TRY(hipEventRecord(e0, st));
+        for (int it = 0; it < iters; ++it)
+            HIP_TRY(lc2e::launch_fused(a, 3, p->info.src_ndim, p->info.high_res_is_moving, n, st));
+        HIP_TRY(hipEventRecord(e1, st));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        if (ms / iters < bms) {
+            bms = ms / iters;
+            best = zc;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    p->use_graph = graph;
+    int rc = lc2e_pair_set_tuning(p, best);
+    if (rc < 0) return rc;
+    if (best_zchunk) *best_zchunk = best;
+    if (best_ms) *best_ms = bms;
+    return LC2E_OK;
 }
 
 int lc2e_pair_get_tuning(lc2e_pair* p, int* zchunk) {

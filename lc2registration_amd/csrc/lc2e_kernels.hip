@@ -53,65 +53,171 @@ struct Unroll<E, E> {
     __device__ __forceinline__ static void run(F&&) {}
 };
 
-__device__ __forceinline__ float load_src(const void* p, int f16, int i) {
-    if (f16) return __half2float(static_cast<const __half*>(p)[i]);
-    return static_cast<const float*>(p)[i];
+template <typename T>
+__device__ __forceinline__ float ld1(const T* p, int i) {
+    if constexpr (sizeof(T) == 2) return __half2float(p[i]);
+    else return p[i];
+}
+
+// The voxels at x0 and x0 + 1 of one row in a single load. Volumes are allocated with a
+// zeroed tail, so x0 + 1 is readable even for the last voxel; when the sample lies on
+// the integer x0 the reference reads x0 twice (ceil == floor), which is restored here.
+template <typename T>
+__device__ __forceinline__ void ld2(const T* p, int i, bool same, float& a, float& b) {
+    if constexpr (sizeof(T) == 2) {
+        __half2 h;
+        __builtin_memcpy(&h, p + i, sizeof(h));
+        a = __low2float(h);
+        b = __high2float(h);
+    } else {
+        float2 f;
+        __builtin_memcpy(&f, p + i, sizeof(f));
+        a = f.x;
+        b = f.y;
+    }
+    if (same) b = a;
+}
+
+__device__ __forceinline__ void to_voxel(const SGeom& s, float px, float py, float pz, float& vx,
+                                         float& vy, float& vz) {
+    if (s.pow2) {  // exact: scaling by a power of two
+        vx = (px - s.ox) * s.rsx;
+        vy = (py - s.oy) * s.rsy;
+        vz = (pz - s.oz) * s.rsz;
+    } else {
+        vx = (px - s.ox) / s.sx;
+        vy = (py - s.oy) / s.sy;
+        vz = (pz - s.oz) / s.sz;
+    }
 }
 
 // Interpolate_3Dkernel sample (interpolator.cu:76-174): L -> M*L -> voxel -> trilinear
 // z, y, x with floor/ceil corners; 0 outside [0, dim-1].
-__device__ __forceinline__ float sample3(const void* src, int f16, const SGeom& s, const float* m,
+template <typename T>
+__device__ __forceinline__ float sample3(const T* __restrict__ src, const SGeom& s, const float* m,
                                          float lx, float ly, float lz) {
     const float px = fmaf(m[2], lz, fmaf(m[1], ly, m[0] * lx)) + m[3];
     const float py = fmaf(m[6], lz, fmaf(m[5], ly, m[4] * lx)) + m[7];
     const float pz = fmaf(m[10], lz, fmaf(m[9], ly, m[8] * lx)) + m[11];
-    const float vx = (px - s.ox) / s.sx;
-    const float vy = (py - s.oy) / s.sy;
-    const float vz = (pz - s.oz) / s.sz;
+    float vx, vy, vz;
+    to_voxel(s, px, py, pz, vx, vy, vz);
     if (!(vx >= 0.0f && vx <= (float)(s.X - 1) && vy >= 0.0f && vy <= (float)(s.Y - 1) &&
           vz >= 0.0f && vz <= (float)(s.Z - 1)))
         return 0.0f;
-    const int x0 = (int)floorf(vx), x1 = (int)ceilf(vx);
-    const int y0 = (int)floorf(vy), y1 = (int)ceilf(vy);
-    const int z0 = (int)floorf(vz), z1 = (int)ceilf(vz);
-    const float rx = vx - (float)x0, ry = vy - (float)y0, rz = vz - (float)z0;
+    const float fx = floorf(vx), fy = floorf(vy), fz = floorf(vz);
+    const int x0 = (int)fx, y0 = (int)fy, z0 = (int)fz;
+    const int y1 = (int)ceilf(vy), z1 = (int)ceilf(vz);
+    const bool xs = (fx == vx);
+    const float rx = vx - fx, ry = vy - fy, rz = vz - fz;
     const int sxy = s.X * s.Y;
     const int r00 = y0 * s.X + z0 * sxy, r01 = y0 * s.X + z1 * sxy;
     const int r10 = y1 * s.X + z0 * sxy, r11 = y1 * s.X + z1 * sxy;
-    float a, b;
-    a = load_src(src, f16, x0 + r00); b = load_src(src, f16, x0 + r01);
-    const float lu = fmaf(rz, b - a, a);
-    a = load_src(src, f16, x1 + r00); b = load_src(src, f16, x1 + r01);
-    const float ru = fmaf(rz, b - a, a);
-    a = load_src(src, f16, x1 + r10); b = load_src(src, f16, x1 + r11);
-    const float rb = fmaf(rz, b - a, a);
-    a = load_src(src, f16, x0 + r10); b = load_src(src, f16, x0 + r11);
-    const float lb = fmaf(rz, b - a, a);
+    float a0, a1, b0, b1;
+    ld2(src, x0 + r00, xs, a0, a1);  // (x0, y0, z0), (x1, y0, z0)
+    ld2(src, x0 + r01, xs, b0, b1);  // (x0, y0, z1), (x1, y0, z1)
+    const float lu = fmaf(rz, b0 - a0, a0);
+    const float ru = fmaf(rz, b1 - a1, a1);
+    ld2(src, x0 + r10, xs, a0, a1);  // (x0, y1, z0), (x1, y1, z0)
+    ld2(src, x0 + r11, xs, b0, b1);  // (x0, y1, z1), (x1, y1, z1)
+    const float lb = fmaf(rz, b0 - a0, a0);
+    const float rb = fmaf(rz, b1 - a1, a1);
     const float yl = fmaf(ry, lb - lu, lu);
     const float yr = fmaf(ry, rb - ru, ru);
     return fmaf(rx, yr - yl, yl);
 }
 
+// The same trilinear sample split in two for software pipelining: issue3() computes the
+// voxel coordinates and starts the four paired loads, finish3() does the lerps once the
+// data has arrived, so a whole z-step of work can run in between.
+struct Pending3 {
+    float a0, a1, b0, b1, c0, c1, d0, d1;
+    float rx, ry, rz;
+    int ok;
+};
+
+template <typename T>
+__device__ __forceinline__ void issue3(const T* __restrict__ src, const SGeom& s, const float* m,
+                                       float lx, float ly, float lz, bool inside, Pending3& p) {
+    p.ok = 0;
+    if (!inside) return;
+    const float px = fmaf(m[2], lz, fmaf(m[1], ly, m[0] * lx)) + m[3];
+    const float py = fmaf(m[6], lz, fmaf(m[5], ly, m[4] * lx)) + m[7];
+    const float pz = fmaf(m[10], lz, fmaf(m[9], ly, m[8] * lx)) + m[11];
+    float vx, vy, vz;
+    to_voxel(s, px, py, pz, vx, vy, vz);
+    if (!(vx >= 0.0f && vx <= (float)(s.X - 1) && vy >= 0.0f && vy <= (float)(s.Y - 1) &&
+          vz >= 0.0f && vz <= (float)(s.Z - 1)))
+        return;
+    const float fx = floorf(vx), fy = floorf(vy), fz = floorf(vz);
+    const int x0 = (int)fx, y0 = (int)fy, z0 = (int)fz;
+    const int y1 = (int)ceilf(vy), z1 = (int)ceilf(vz);
+    const bool xs = (fx == vx);
+    p.rx = vx - fx;
+    p.ry = vy - fy;
+    p.rz = vz - fz;
+    const int sxy = s.X * s.Y;
+    ld2(src, x0 + y0 * s.X + z0 * sxy, xs, p.a0, p.a1);
+    ld2(src, x0 + y0 * s.X + z1 * sxy, xs, p.b0, p.b1);
+    ld2(src, x0 + y1 * s.X + z0 * sxy, xs, p.c0, p.c1);
+    ld2(src, x0 + y1 * s.X + z1 * sxy, xs, p.d0, p.d1);
+    p.ok = 1;
+}
+
+__device__ __forceinline__ float finish3(const Pending3& p) {
+    if (!p.ok) return 0.0f;
+    const float lu = fmaf(p.rz, p.b0 - p.a0, p.a0);
+    const float ru = fmaf(p.rz, p.b1 - p.a1, p.a1);
+    const float lb = fmaf(p.rz, p.d0 - p.c0, p.c0);
+    const float rb = fmaf(p.rz, p.d1 - p.c1, p.c1);
+    const float yl = fmaf(p.ry, lb - lu, lu);
+    const float yr = fmaf(p.ry, rb - ru, ru);
+    return fmaf(p.rx, yr - yl, yl);
+}
+
 // Interpolate_2Dkernel sample (interpolator.cu:186-230): bilinear, y then x. The 2x3
 // matrix sits in the 3x4 layout as m[0], m[1], m[3] / m[4], m[5], m[7].
-__device__ __forceinline__ float sample2(const void* src, int f16, const SGeom& s, const float* m,
+template <typename T>
+__device__ __forceinline__ float sample2(const T* __restrict__ src, const SGeom& s, const float* m,
                                          float lx, float ly) {
     const float px = fmaf(m[1], ly, m[0] * lx) + m[3];
     const float py = fmaf(m[5], ly, m[4] * lx) + m[7];
-    const float vx = (px - s.ox) / s.sx;
-    const float vy = (py - s.oy) / s.sy;
+    float vx, vy;
+    if (s.pow2) {
+        vx = (px - s.ox) * s.rsx;
+        vy = (py - s.oy) * s.rsy;
+    } else {
+        vx = (px - s.ox) / s.sx;
+        vy = (py - s.oy) / s.sy;
+    }
     if (!(vx >= 0.0f && vx <= (float)(s.X - 1) && vy >= 0.0f && vy <= (float)(s.Y - 1)))
         return 0.0f;
-    const int x0 = (int)floorf(vx), x1 = (int)ceilf(vx);
-    const int y0 = (int)floorf(vy), y1 = (int)ceilf(vy);
-    const float ry = vy - floorf(vy);
-    float a = load_src(src, f16, x0 + y0 * s.X), b = load_src(src, f16, x0 + y1 * s.X);
-    const float xl = fmaf(ry, b - a, a);
-    a = load_src(src, f16, x1 + y0 * s.X);
-    b = load_src(src, f16, x1 + y1 * s.X);
-    const float xr = fmaf(ry, b - a, a);
-    const float rx = vx - floorf(vx);
+    const float fx = floorf(vx), fy = floorf(vy);
+    const int x0 = (int)fx, y0 = (int)fy, y1 = (int)ceilf(vy);
+    const bool xs = (fx == vx);
+    const float ry = vy - fy;
+    float a0, a1, b0, b1;
+    ld2(src, x0 + y0 * s.X, xs, a0, a1);
+    ld2(src, x0 + y1 * s.X, xs, b0, b1);
+    const float xl = fmaf(ry, b0 - a0, a0);
+    const float xr = fmaf(ry, b1 - a1, a1);
+    const float rx = vx - fx;
     return fmaf(rx, xr - xl, xl);
+}
+
+// FP64 reciprocal and reciprocal square root: hardware estimate + two Newton steps
+// (relative error a few ulp), instead of the long IEEE division / sqrt sequences.
+__device__ __forceinline__ double rcp64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
+
+__device__ __forceinline__ double rsq64(double x) {
+    double r = __builtin_amdgcn_rsq(x);
+    r = r * fma(-0.5 * x * r, r, 1.5);
+    r = r * fma(-0.5 * x * r, r, 1.5);
+    return r;
 }
 
 // s^T C^+ s for the two-column centred fit with the rank rule shared with the oracle.
@@ -123,7 +229,7 @@ __device__ __forceinline__ double explained_var(double a, double b, double c, do
     if (gconst) { b = 0.0; c = 0.0; q = 0.0; }
     const double det = a * b - c * c;
     if (a > 0.0 && b > 0.0 && det > kTauColl * a * b)
-        return (b * p * p - 2.0 * c * p * q + a * q * q) / det;
+        return (b * p * p - 2.0 * c * p * q + a * q * q) * rcp64(det);
     const double tr = a + b;
     if (tr > 0.0) {
         const double u = sqrt(a) * p + (c < 0.0 ? -1.0 : 1.0) * sqrt(b) * q;
@@ -132,33 +238,56 @@ __device__ __forceinline__ double explained_var(double a, double b, double c, do
     return 0.0;
 }
 
-// Patch sums -> (w, measure) with the reference's thresholds
-// (cu_lc2_3d.cu:258-284 weight, :596-611 measure). Returns false if w = 0.
+// Weight and measure from the centred energy of U (cyy) and the explained part e
+// (cu_lc2_3d.cu:258-284 weight, :596-611 measure): w = sqrt(cyy/(N-1)),
+// measure = w - var_res / w with var_res = (cyy - e)/(N-1); 0 if not > 0.
+__device__ __forceinline__ void weight_measure(double cyy, double e, double inv_n1, double& w,
+                                               double& meas) {
+    const double var_u = cyy * inv_n1;
+    const double rw = rsq64(var_u);  // 1 / w
+    w = var_u * rw;
+    e = fmin(fmax(e, 0.0), cyy);
+    meas = w - ((cyy - e) * inv_n1) * rw;
+    if (!(meas > 0.0) || isinf(meas)) meas = 0.0;
+}
+
+// Channels I, G, U, II, GG, IG, UU, UI, UG and the count of U > 1e-6. false where w = 0.
 __device__ __forceinline__ bool combine(const double* S, int cnt, int N, double& w_out,
                                         double& m_out) {
     if (cnt != N) return false;
-    const double n = (double)N;
+    const double inv_n = 1.0 / (double)N, inv_n1 = 1.0 / (double)(N - 1);
     const double sy = S[2];
-    const double cyy = S[6] - sy * sy / n;
-    const double var_u = cyy / (n - 1.0);
-    if (!(var_u > kZeroThreshold)) return false;
-    const double w = sqrt(var_u);
+    const double cyy = S[6] - sy * sy * inv_n;
+    if (!(cyy * inv_n1 > kZeroThreshold)) return false;
     const double si = S[0], sg = S[1];
-    const double cii = S[3] - si * si / n;
-    const double cgg = S[4] - sg * sg / n;
-    const double cig = S[5] - si * sg / n;
-    const double cyi = S[7] - sy * si / n;
-    const double cyg = S[8] - sy * sg / n;
-    double e = explained_var(cii, cgg, cig, cyi, cyg, S[3], S[4]);
-    e = fmin(fmax(e, 0.0), cyy);
-    const double var_res = (cyy - e) / (n - 1.0);
-    double meas = w - var_res / w;
-    if (!(meas > 0.0) || isinf(meas)) meas = 0.0;
-    w_out = w;
-    m_out = meas;
+    const double cii = S[3] - si * si * inv_n;
+    const double cgg = S[4] - sg * sg * inv_n;
+    const double cig = S[5] - si * sg * inv_n;
+    const double cyi = S[7] - sy * si * inv_n;
+    const double cyg = S[8] - sy * sg * inv_n;
+    const double e = explained_var(cii, cgg, cig, cyi, cyg, S[3], S[4]);
+    weight_measure(cyy, e, inv_n1, w_out, m_out);
     return true;
 }
 
+// Static ultrasound: channels I, G, II, GG, IG, UI, UG; mu = patch mean of U,
+// cyy = its centred energy (< 0 where w = 0).
+__device__ __forceinline__ bool combine_static_u(const double* S, double mu, double cyy, int N,
+                                                 double& w_out, double& m_out) {
+    if (!(cyy >= 0.0)) return false;
+    const double inv_n = 1.0 / (double)N, inv_n1 = 1.0 / (double)(N - 1);
+    const double si = S[0], sg = S[1];
+    const double cii = S[2] - si * si * inv_n;
+    const double cgg = S[3] - sg * sg * inv_n;
+    const double cig = S[4] - si * sg * inv_n;
+    const double cyi = S[5] - mu * si;
+    const double cyg = S[6] - mu * sg;
+    const double e = explained_var(cii, cgg, cig, cyi, cyg, S[2], S[3]);
+    weight_measure(cyy, e, inv_n1, w_out, m_out);
+    return true;
+}
+
+template <int NTHR>
 __device__ __forceinline__ void block_reduce_store(double am, double aw, double* s_red,
                                                    double* dst) {
     for (int off = 32; off > 0; off >>= 1) {
@@ -173,7 +302,7 @@ __device__ __forceinline__ void block_reduce_store(double am, double aw, double*
     __syncthreads();
     if (threadIdx.x == 0) {
         double m = 0.0, w = 0.0;
-        for (int i = 0; i < kThreads / 64; ++i) {
+        for (int i = 0; i < NTHR / 64; ++i) {
             m += s_red[2 * i];
             w += s_red[2 * i + 1];
         }
@@ -183,118 +312,173 @@ __device__ __forceinline__ void block_reduce_store(double am, double aw, double*
 }
 
 // Products of one position into the channel planes of the LDS buffer.
+// NC = 9 (+count): I, G, U, II, GG, IG, UU, UI, UG.  NC = 7: I, G, II, GG, IG, UI, UG.
+template <int NC>
 __device__ __forceinline__ void store_products(double* s_prod, int* s_cnt, int np, int i, float I,
                                                float G, float U) {
     const double di = I, dg = G, du = U;
-    s_prod[0 * np + i] = di;
-    s_prod[1 * np + i] = dg;
-    s_prod[2 * np + i] = du;
-    s_prod[3 * np + i] = di * di;
-    s_prod[4 * np + i] = dg * dg;
-    s_prod[5 * np + i] = di * dg;
-    s_prod[6 * np + i] = du * du;
-    s_prod[7 * np + i] = du * di;
-    s_prod[8 * np + i] = du * dg;
-    s_cnt[i] = (du > kZeroThreshold) ? 1 : 0;
+    if (NC == 9) {
+        s_prod[0 * np + i] = di;
+        s_prod[1 * np + i] = dg;
+        s_prod[2 * np + i] = du;
+        s_prod[3 * np + i] = di * di;
+        s_prod[4 * np + i] = dg * dg;
+        s_prod[5 * np + i] = di * dg;
+        s_prod[6 * np + i] = du * du;
+        s_prod[7 * np + i] = du * di;
+        s_prod[8 * np + i] = du * dg;
+        s_cnt[i] = (du > kZeroThreshold) ? 1 : 0;
+    } else {
+        s_prod[0 * np + i] = di;
+        s_prod[1 * np + i] = dg;
+        s_prod[2 * np + i] = di * di;
+        s_prod[3 * np + i] = dg * dg;
+        s_prod[4 * np + i] = di * dg;
+        s_prod[5 * np + i] = du * di;
+        s_prod[6 * np + i] = du * dg;
+    }
 }
 
-// Sliding box sums along x (rows of width px -> kTile outputs) then along y.
-// s_prod[c][py*px] -> s_xp[c][py*kTile] -> s_out[c][kTile*kTile] (s_out may alias s_prod).
+// Sliding box sums of width w along x then y, for NC FP64 channels and (CNT) one integer
+// count channel, over a px x py region into TX x TY outputs:
+//   s_prod[c][py][px] -> s_xp[c][py][TX] -> s_out[c][TY][TX]   (s_out may alias s_prod).
+// Rows (x pass) and columns (y pass) are cut into runs of 16 outputs, one task each.
+template <typename V>
+__device__ __forceinline__ void slide16(const V* in, int stride, int w, V* out, int ostride) {
+    V acc = 0;
+    for (int k = 0; k < w; ++k) acc += in[k * stride];
+    out[0] = acc;
+#pragma unroll
+    for (int o = 1; o < 16; ++o) {
+        acc += in[(o + w - 1) * stride] - in[(o - 1) * stride];
+        out[o * ostride] = acc;
+    }
+}
+
+template <int NC, bool CNT, int TX, int TY, int NTHR>
 __device__ __forceinline__ void box_xy(const double* s_prod, const int* s_cnt, double* s_xp,
-                                       int* s_xcnt, double* s_out, int* s_ocnt, int px, int w,
-                                       int tid) {
-    const int np = px * px;
-    // x pass: task = (channel, row)
-    for (int t = tid; t < (NCH + 1) * px; t += kThreads) {
-        const int c = t / px, row = t - c * px;
-        if (c < NCH) {
-            const double* in = s_prod + c * np + row * px;
-            double* out = s_xp + c * kTile * px + row * kTile;
-            double acc = 0.0;
-            for (int k = 0; k < w; ++k) acc += in[k];
-            out[0] = acc;
-            for (int xo = 1; xo < kTile; ++xo) {
-                acc += in[xo + w - 1] - in[xo - 1];
-                out[xo] = acc;
-            }
-        } else {
-            const int* in = s_cnt + row * px;
-            int* out = s_xcnt + row * kTile;
-            int acc = 0;
-            for (int k = 0; k < w; ++k) acc += in[k];
-            out[0] = acc;
-            for (int xo = 1; xo < kTile; ++xo) {
-                acc += in[xo + w - 1] - in[xo - 1];
-                out[xo] = acc;
-            }
-        }
+                                       int* s_xcnt, double* s_out, int* s_ocnt, int px, int py,
+                                       int w, int tid) {
+    static_assert(TX % 16 == 0 && TY % 16 == 0, "tile edges are multiples of 16");
+    constexpr int NT = NC + (CNT ? 1 : 0);
+    constexpr int SX = TX / 16, SY = TY / 16;
+    const int np = px * py;
+    for (int t = tid; t < NT * py * SX; t += NTHR) {
+        const int seg = t % SX, rest = t / SX;
+        const int row = rest % py, c = rest / py;
+        if (c < NC)
+            slide16(s_prod + c * np + row * px + seg * 16, 1, w,
+                    s_xp + (c * py + row) * TX + seg * 16, 1);
+        else
+            slide16(s_cnt + row * px + seg * 16, 1, w, s_xcnt + row * TX + seg * 16, 1);
     }
     __syncthreads();
-    // y pass: task = (channel, column)
-    for (int t = tid; t < (NCH + 1) * kTile; t += kThreads) {
-        const int c = t / kTile, col = t - c * kTile;
-        if (c < NCH) {
-            const double* in = s_xp + c * kTile * px + col;
-            double* out = s_out + c * kTile * kTile + col;
-            double acc = 0.0;
-            for (int k = 0; k < w; ++k) acc += in[k * kTile];
-            out[0] = acc;
-            for (int yo = 1; yo < kTile; ++yo) {
-                acc += in[(yo + w - 1) * kTile] - in[(yo - 1) * kTile];
-                out[yo * kTile] = acc;
-            }
-        } else {
-            const int* in = s_xcnt + col;
-            int* out = s_ocnt + col;
-            int acc = 0;
-            for (int k = 0; k < w; ++k) acc += in[k * kTile];
-            out[0] = acc;
-            for (int yo = 1; yo < kTile; ++yo) {
-                acc += in[(yo + w - 1) * kTile] - in[(yo - 1) * kTile];
-                out[yo * kTile] = acc;
-            }
-        }
+    for (int t = tid; t < NT * TX * SY; t += NTHR) {
+        const int seg = t % SY, rest = t / SY;
+        const int col = rest % TX, c = rest / TX;
+        if (c < NC)
+            slide16(s_xp + c * py * TX + seg * 16 * TX + col, TX, w,
+                    s_out + c * TX * TY + seg * 16 * TX + col, TX);
+        else
+            slide16(s_xcnt + seg * 16 * TX + col, TX, w, s_ocnt + seg * 16 * TX + col, TX);
     }
     __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
-// 3D fused kernel
+// 3D fused kernel: a TX x TY column of output voxels over `zchunk` planes of one pose.
+// MOVI: the moving image is the high-resolution I (warped + Sobel here), the fixed
+// ultrasound U is static with precomputed patch statistics (7 channels). Otherwise U is
+// warped here and I, |grad I| are static (9 channels + count).
+// Global loads are software-pipelined one z-step ahead: the gathers of the next warped
+// plane, the static image values of the next products plane and the statistics of the
+// next output plane are issued early in a step and consumed in the following one.
 // ---------------------------------------------------------------------------
-template <int R, bool MOVI>
-__global__ __launch_bounds__(kThreads, 2) void k_fused3d(FusedArgs a) {
+template <int R, bool MOVI, typename T, int TX, int TY>
+__global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
+    constexpr int NTHR = TX * TY;
     constexpr int W = 2 * R + 1;
-    constexpr int PX = kTile + 2 * R;  // patch-support region edge
-    constexpr int NP = PX * PX;
-    constexpr int IXE = PX + 2;        // Sobel input region edge
-    constexpr int NI = MOVI ? IXE * IXE : NP;
-    constexpr int PPT = (NP + kThreads - 1) / kThreads;
+    constexpr int PX = TX + 2 * R, PY = TY + 2 * R;  // patch-support region
+    constexpr int NP = PX * PY;
+    constexpr int IXE = PX + 2, IYE = PY + 2;        // Sobel input region
+    constexpr int NI = MOVI ? IXE * IYE : NP;        // warped positions per plane
+    constexpr int SPT = (NI + NTHR - 1) / NTHR;      // warped samples per thread
+    constexpr int PPT = (NP + NTHR - 1) / NTHR;      // products positions per thread
     constexpr int N3 = W * W * W;
     constexpr int LAG = MOVI ? 2 : 0;
+    constexpr int NC = MOVI ? 7 : 9;
+    constexpr bool CNT = !MOVI;
 
     __shared__ float s_in[NI];
-    __shared__ float s_s[MOVI ? PX * IXE : 1];
-    __shared__ float s_d[MOVI ? PX * IXE : 1];
-    __shared__ double s_prod[NCH * NP];
-    __shared__ int s_cnt[NP];
-    __shared__ double s_xp[NCH * kTile * PX];
-    __shared__ int s_xcnt[kTile * PX];
-    __shared__ double s_red[2 * kThreads / 64];
+    __shared__ float s_s[MOVI ? PX * IYE : 1];
+    __shared__ float s_d[MOVI ? PX * IYE : 1];
+    __shared__ double s_prod[NC * NP];
+    __shared__ int s_cnt[CNT ? NP : 1];
+    __shared__ double s_xp[NC * TX * PY];
+    __shared__ int s_xcnt[CNT ? TX * PY : 1];
+    __shared__ double s_red[2 * NTHR / 64];
 
     const int tid = threadIdx.x;
     const int tx = blockIdx.x % a.ntx, ty = blockIdx.x / a.ntx;
-    const int x0 = R + tx * kTile, y0 = R + ty * kTile;
+    const int x0 = R + tx * TX, y0 = R + ty * TY;
     const int z0 = R + blockIdx.y * a.zchunk;
     const int zend = min(z0 + a.zchunk, a.v.Z - R);
     const int nout = zend - z0;
     const int pose = blockIdx.z;
     const int X = a.v.X, Y = a.v.Y, Z = a.v.Z;
+    const T* __restrict__ src = static_cast<const T*>(a.src);
     float m[12];
     {
         const float* mp = a.mats + (size_t)pose * a.mats_per_pose * 12;
 #pragma unroll
         for (int k = 0; k < 12; ++k) m[k] = mp[k];
     }
+    // warped plane of step s: MOVI -> zi = z0-R-1+s (Sobel input), else zp = z0-R+s
+    const int zw0 = MOVI ? z0 - R - 1 : z0 - R;
+    const int wex = MOVI ? IXE : PX;     // warped region edge along x
+    const int wofs = MOVI ? R + 1 : R;   // warped region starts at x0 - wofs, y0 - wofs
+    const int nsteps = nout + 2 * R + LAG;
+
+    Pending3 pend[SPT];
+    auto issue_plane = [&](int zi) {
+        const bool zin = zi >= 0 && zi < Z;
+        const float lz = fmaf((float)zi, a.v.sz, a.v.oz);
+#pragma unroll
+        for (int j = 0; j < SPT; ++j) {
+            const int i = tid + j * NTHR;
+            const int lx = i % wex, ly = i / wex;
+            const int gx = x0 - wofs + lx, gy = y0 - wofs + ly;
+            const bool inside = i < NI && zin && gx >= 0 && gx < X && gy >= 0 && gy < Y;
+            issue3(src, a.s, m, fmaf((float)gx, a.v.sx, a.v.ox), fmaf((float)gy, a.v.sy, a.v.oy),
+                   lz, inside, pend[j]);
+        }
+    };
+    // static image values (U if MOVI, else I and |grad I|) of the products positions
+    float pre0[PPT], pre1[PPT];
+    auto issue_static = [&](int zp) {
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            const int i = tid + j * NTHR;
+            const int lx = i % PX, ly = i / PX;
+            const int gx = x0 - R + lx, gy = y0 - R + ly;
+            const bool in = i < NP && gx < X && gy < Y && zp >= 0 && zp < Z;
+            const int vidx = (zp * Y + gy) * X + gx;
+            pre0[j] = in ? a.stat_img[vidx] : 0.0f;
+            if (!MOVI) pre1[j] = in ? a.stat_grad[vidx] : 0.0f;
+        }
+    };
+    // static ultrasound statistics of this thread's output voxel
+    const int ox = tid % TX, oy = tid / TX;
+    const bool col_valid = (x0 + ox < X - R) && (y0 + oy < Y - R);
+    double mu_pre = 0.0, cyy_pre = -1.0;
+    auto issue_stats = [&](int zo) {
+        if (MOVI) {
+            const int vo = (zo * Y + (y0 + oy)) * X + (x0 + ox);
+            const bool ok = col_valid && zo >= z0 && zo < zend;
+            mu_pre = ok ? a.ustat_mean[vo] : 0.0;
+            cyy_pre = ok ? a.ustat_cyy[vo] : -1.0;
+        }
+    };
 
     float tA[PPT][3], tBx[PPT][3], tBy[PPT][3], Iprev[PPT], Icur[PPT];
 #pragma unroll
@@ -303,62 +487,53 @@ __global__ __launch_bounds__(kThreads, 2) void k_fused3d(FusedArgs a) {
         for (int k = 0; k < 3; ++k) { tA[j][k] = 0.f; tBx[j][k] = 0.f; tBy[j][k] = 0.f; }
         Iprev[j] = 0.f;
         Icur[j] = 0.f;
+        pre0[j] = 0.f;
+        pre1[j] = 0.f;
     }
-    double ring[W][NCH];
+    double ring[W][NC];
     int ringc[W];
-    double S[NCH];
+    double S[NC];
     int SC = 0;
 #pragma unroll
     for (int k = 0; k < W; ++k) {
         ringc[k] = 0;
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) ring[k][c] = 0.0;
+        for (int c = 0; c < NC; ++c) ring[k][c] = 0.0;
     }
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) S[c] = 0.0;
-
-    const int ox = tid % kTile, oy = tid / kTile;
-    const bool col_valid = (x0 + ox < X - R) && (y0 + oy < Y - R);
+    for (int c = 0; c < NC; ++c) S[c] = 0.0;
     double acc_m = 0.0, acc_w = 0.0;
 
-    const int nsteps = nout + 2 * R + LAG;
+    issue_plane(zw0);
+    issue_static(z0 - R);      // products plane of the first products step
+    issue_stats(z0);           // first output plane
+
     for (int s0 = 0; s0 < nsteps; s0 += W) {
         // W consecutive steps with compile-time ring slots (k), so the z ring stays in VGPRs.
         Unroll<0, W>::run([&](auto kc) {
             constexpr int k = decltype(kc)::value;
             const int s = s0 + k;
             if (s >= nsteps) return;  // block-uniform
-            // 1. warp one plane into LDS
-            if (MOVI) {
-                const int zi = z0 - R - 1 + s;
-                const bool zin = zi >= 0 && zi < Z;
-                const float lz = fmaf((float)zi, a.v.sz, a.v.oz);
-                for (int i = tid; i < NI; i += kThreads) {
-                    const int lx = i % IXE, ly = i / IXE;
-                    const int gx = x0 - R - 1 + lx, gy = y0 - R - 1 + ly;
-                    float val = 0.0f;
-                    if (zin && gx >= 0 && gx < X && gy >= 0 && gy < Y)
-                        val = sample3(a.src, a.src_f16, a.s, m, fmaf((float)gx, a.v.sx, a.v.ox),
-                                      fmaf((float)gy, a.v.sy, a.v.oy), lz);
-                    s_in[i] = val;
-                }
-            } else {
-                const int zp = z0 - R + s;
-                const float lz = fmaf((float)zp, a.v.sz, a.v.oz);
-                for (int i = tid; i < NI; i += kThreads) {
-                    const int lx = i % PX, ly = i / PX;
-                    const int gx = x0 - R + lx, gy = y0 - R + ly;
-                    float val = 0.0f;
-                    if (gx < X && gy < Y)
-                        val = sample3(a.src, a.src_f16, a.s, m, fmaf((float)gx, a.v.sx, a.v.ox),
-                                      fmaf((float)gy, a.v.sy, a.v.oy), lz);
-                    s_in[i] = val;
-                }
+            const int q = s - LAG;
+            const int zp = z0 - R + q;
+            // 1. finish the plane issued last step, start the next one
+#pragma unroll
+            for (int j = 0; j < SPT; ++j) {
+                const int i = tid + j * NTHR;
+                if (i < NI) s_in[i] = finish3(pend[j]);
             }
+            if (s + 1 < nsteps) issue_plane(zw0 + s + 1);
+            float cur0[PPT], cur1[PPT];
+#pragma unroll
+            for (int j = 0; j < PPT; ++j) {
+                cur0[j] = pre0[j];
+                cur1[j] = pre1[j];
+            }
+            if (q >= 0 && q + 1 < nout + 2 * R) issue_static(zp + 1);
             __syncthreads();
             // 2. Sobel row pass (smooth s and difference d along x)
             if (MOVI) {
-                for (int i = tid; i < PX * IXE; i += kThreads) {
+                for (int i = tid; i < PX * IYE; i += NTHR) {
                     const int lx = i % PX, ly = i / PX;
                     const float* row = s_in + ly * IXE + lx;
                     const float l = row[0], c = row[1], r = row[2];
@@ -368,11 +543,9 @@ __global__ __launch_bounds__(kThreads, 2) void k_fused3d(FusedArgs a) {
                 __syncthreads();
             }
             // 3. column pass into the 3-plane ring, |grad| and products of plane zp
-            const int q = s - LAG;
-            const int zp = z0 - R + q;
 #pragma unroll
             for (int j = 0; j < PPT; ++j) {
-                const int i = tid + j * kThreads;
+                const int i = tid + j * NTHR;
                 if (i < NP) {
                     const int lx = i % PX, ly = i / PX;
                     if (MOVI) {
@@ -390,9 +563,6 @@ __global__ __launch_bounds__(kThreads, 2) void k_fused3d(FusedArgs a) {
                         Icur[j] = s_in[(ly + 1) * IXE + lx + 1];
                     }
                     if (q >= 0) {
-                        const int gx = x0 - R + lx, gy = y0 - R + ly;
-                        const bool in = gx < X && gy < Y;
-                        const int vidx = (zp * Y + gy) * X + gx;
                         float I, G, U;
                         if (MOVI) {
                             const float gxv = (tBx[j][0] + 2.0f * tBx[j][1]) + tBx[j][2];
@@ -400,40 +570,50 @@ __global__ __launch_bounds__(kThreads, 2) void k_fused3d(FusedArgs a) {
                             const float gzv = tA[j][2] - tA[j][0];
                             G = sqrtf(fmaf(gzv, gzv, fmaf(gyv, gyv, gxv * gxv)));
                             I = Iprev[j];
-                            U = in ? a.stat_img[vidx] : 0.0f;
+                            U = cur0[j];
                         } else {
                             U = s_in[i];
-                            I = in ? a.stat_img[vidx] : 0.0f;
-                            G = in ? a.stat_grad[vidx] : 0.0f;
+                            I = cur0[j];
+                            G = cur1[j];
                         }
-                        store_products(s_prod, s_cnt, NP, i, I, G, U);
+                        store_products<NC>(s_prod, s_cnt, NP, i, I, G, U);
                     }
                 }
             }
             __syncthreads();  // s_in / s_s / s_d reads done; products visible
             if (q < 0) return;  // block-uniform: Sobel ring warm-up only
             // 4. in-plane box sums
-            box_xy(s_prod, s_cnt, s_xp, s_xcnt, s_prod, s_cnt, PX, W, tid);
+            box_xy<NC, CNT, TX, TY, NTHR>(s_prod, s_cnt, s_xp, s_xcnt, s_prod, s_cnt, PX, PY, W,
+                                          tid);
             // 5. z ring of plane sums, running patch sums, combine
             constexpr int slot = (k - LAG + W) % W;
-            const int cc = s_cnt[tid];
-            SC += cc - ringc[slot];
-            ringc[slot] = cc;
+            if (CNT) {
+                const int cc = s_cnt[tid];
+                SC += cc - ringc[slot];
+                ringc[slot] = cc;
+            }
 #pragma unroll
-            for (int c = 0; c < NCH; ++c) {
-                const double v = s_prod[c * kTile * kTile + tid];
+            for (int c = 0; c < NC; ++c) {
+                const double v = s_prod[c * TX * TY + tid];
                 S[c] += v - ring[slot][c];
                 ring[slot][c] = v;
             }
-            if (q >= 2 * R && col_valid) {
-                double w, meas;
-                if (combine(S, SC, N3, w, meas)) {
-                    acc_m += meas;
-                    acc_w += w;
-                    if (a.wmap) {
-                        const int vo = ((zp - R) * Y + (y0 + oy)) * X + (x0 + ox);
-                        a.wmap[vo] = w;
-                        a.mmap[vo] = meas;
+            if (q >= 2 * R) {
+                const double mu = mu_pre, cyy = cyy_pre;
+                if (q + 1 < nout + 2 * R) issue_stats(zp - R + 1);
+                if (col_valid) {
+                    double w, meas;
+                    bool ok;
+                    if (MOVI) ok = combine_static_u(S, mu, cyy, N3, w, meas);
+                    else ok = combine(S, SC, N3, w, meas);
+                    if (ok) {
+                        acc_m += meas;
+                        acc_w += w;
+                        if (a.wmap) {
+                            const int vo = ((zp - R) * Y + (y0 + oy)) * X + (x0 + ox);
+                            a.wmap[vo] = w;
+                            a.mmap[vo] = meas;
+                        }
                     }
                 }
             }
@@ -441,21 +621,23 @@ __global__ __launch_bounds__(kThreads, 2) void k_fused3d(FusedArgs a) {
     }
     double* dst = a.partials + ((size_t)pose * a.blocks_per_pose +
                                 (size_t)blockIdx.y * (a.ntx * a.nty) + blockIdx.x) * 2;
-    block_reduce_store(acc_m, acc_w, s_red, dst);
+    block_reduce_store<NTHR>(acc_m, acc_w, s_red, dst);
 }
 
 // ---------------------------------------------------------------------------
 // 2D fused kernel: one 16x16 output tile of one frame (plane) of one pose.
 // Dynamic LDS sized for the runtime patch radius.
 // ---------------------------------------------------------------------------
-template <bool MOVI, bool SRC3D>
+template <bool MOVI, bool SRC3D, typename T>
 __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
     extern __shared__ double s_dyn[];
+    constexpr int NC = MOVI ? 7 : 9;
+    constexpr bool CNT = !MOVI;
     const int R = a.r, W = 2 * R + 1, PX = kTile + 2 * R, NP = PX * PX, IXE = PX + 2;
     const int NI = MOVI ? IXE * IXE : NP;
-    double* s_prod = s_dyn;                                   // NCH*NP
-    double* s_xp = s_prod + NCH * NP;                         // NCH*kTile*PX
-    double* s_red = s_xp + NCH * kTile * PX;                  // 8
+    double* s_prod = s_dyn;                                   // NC*NP
+    double* s_xp = s_prod + NC * NP;                          // NC*kTile*PX
+    double* s_red = s_xp + NC * kTile * PX;                   // 8
     float* s_in = reinterpret_cast<float*>(s_red + 8);        // NI
     float* s_s = s_in + NI;                                   // PX*IXE
     float* s_d = s_s + PX * IXE;                              // PX*IXE
@@ -477,7 +659,7 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
         for (int k = 0; k < 12; ++k) m[k] = mp[k];
     }
     const float lz = a.v.oz;
-    const void* src = a.src;
+    const T* __restrict__ src = static_cast<const T*>(a.src);
     // 1. warp
     if (MOVI) {
         for (int i = tid; i < NI; i += kThreads) {
@@ -486,8 +668,7 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
             float val = 0.0f;
             if (gx >= 0 && gx < X && gy >= 0 && gy < Y) {
                 const float Lx = fmaf((float)gx, a.v.sx, a.v.ox), Ly = fmaf((float)gy, a.v.sy, a.v.oy);
-                val = SRC3D ? sample3(src, a.src_f16, a.s, m, Lx, Ly, lz)
-                            : sample2(src, a.src_f16, a.s, m, Lx, Ly);
+                val = SRC3D ? sample3(src, a.s, m, Lx, Ly, lz) : sample2(src, a.s, m, Lx, Ly);
             }
             s_in[i] = val;
         }
@@ -498,8 +679,7 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
             float val = 0.0f;
             if (gx < X && gy < Y) {
                 const float Lx = fmaf((float)gx, a.v.sx, a.v.ox), Ly = fmaf((float)gy, a.v.sy, a.v.oy);
-                val = SRC3D ? sample3(src, a.src_f16, a.s, m, Lx, Ly, lz)
-                            : sample2(src, a.src_f16, a.s, m, Lx, Ly);
+                val = SRC3D ? sample3(src, a.s, m, Lx, Ly, lz) : sample2(src, a.s, m, Lx, Ly);
             }
             s_in[i] = val;
         }
@@ -535,22 +715,26 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
             I = in ? a.stat_img[vidx] : 0.0f;
             G = in ? a.stat_grad[vidx] : 0.0f;
         }
-        store_products(s_prod, s_cnt, NP, i, I, G, U);
+        store_products<NC>(s_prod, s_cnt, NP, i, I, G, U);
     }
     __syncthreads();
-    box_xy(s_prod, s_cnt, s_xp, s_xcnt, s_prod, s_cnt, PX, W, tid);
+    box_xy<NC, CNT, kTile, kTile, kThreads>(s_prod, s_cnt, s_xp, s_xcnt, s_prod, s_cnt, PX, PX, W,
+                                            tid);
     const int ox = tid % kTile, oy = tid / kTile;
     double acc_m = 0.0, acc_w = 0.0;
     if ((x0 + ox < X - R) && (y0 + oy < Y - R)) {
-        double S[NCH];
+        double S[NC];
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) S[c] = s_prod[c * kTile * kTile + tid];
+        for (int c = 0; c < NC; ++c) S[c] = s_prod[c * kTile * kTile + tid];
         double w, meas;
-        if (combine(S, s_cnt[tid], W * W, w, meas)) {
+        bool ok;
+        const int vo = plane + (y0 + oy) * X + (x0 + ox);
+        if (MOVI) ok = combine_static_u(S, a.ustat_mean[vo], a.ustat_cyy[vo], W * W, w, meas);
+        else ok = combine(S, s_cnt[tid], W * W, w, meas);
+        if (ok) {
             acc_m = meas;
             acc_w = w;
             if (a.wmap) {
-                const int vo = plane + (y0 + oy) * X + (x0 + ox);
                 a.wmap[vo] = w;
                 a.mmap[vo] = meas;
             }
@@ -558,7 +742,7 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
     }
     double* dst = a.partials + ((size_t)pose * a.blocks_per_pose +
                                 (size_t)frame * (a.ntx * a.nty) + blockIdx.x) * 2;
-    block_reduce_store(acc_m, acc_w, s_red, dst);
+    block_reduce_store<kThreads>(acc_m, acc_w, s_red, dst);
 }
 
 // Per-pose fixed-order reduction of the work-group partials -> (sum m, sum w, LC2)
@@ -593,8 +777,8 @@ __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ parti
     }
 }
 
-__global__ void k_warp_static(const void* fixed, int f16, SGeom fg, VGeom v, int mode,
-                              float* out) {
+template <typename T>
+__global__ void k_warp_static(const T* fixed, SGeom fg, VGeom v, int mode, float* out) {
     // mode 0: 3D grid, trilinear; 1: 2D grid, bilinear per frame (frames = v.Z)
     const long long n = (long long)v.X * v.Y * v.Z;
     const float ident[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
@@ -603,20 +787,18 @@ __global__ void k_warp_static(const void* fixed, int f16, SGeom fg, VGeom v, int
         const int x = (int)(i % v.X), y = (int)((i / v.X) % v.Y), z = (int)(i / ((long long)v.X * v.Y));
         const float lx = fmaf((float)x, v.sx, v.ox), ly = fmaf((float)y, v.sy, v.oy);
         float val;
-        if (mode == 0) {
-            val = sample3(fixed, f16, fg, ident, lx, ly, fmaf((float)z, v.sz, v.oz));
-        } else {
-            const int off = z * fg.X * fg.Y;
-            const void* fp = f16 ? (const void*)(static_cast<const __half*>(fixed) + off)
-                                 : (const void*)(static_cast<const float*>(fixed) + off);
-            val = sample2(fp, f16, fg, ident, lx, ly);
-        }
+        if (mode == 0)
+            val = sample3(fixed, fg, ident, lx, ly, fmaf((float)z, v.sz, v.oz));
+        else
+            val = sample2(fixed + (size_t)z * fg.X * fg.Y, fg, ident, lx, ly);
         out[i] = val;
     }
 }
 
+template <typename T>
 __global__ void k_warp_moving(FusedArgs a, int grid3, int src3, float* out) {
     const VGeom v = a.v;
+    const T* src = static_cast<const T*>(a.src);
     const long long n = (long long)v.X * v.Y * v.Z;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
@@ -626,8 +808,47 @@ __global__ void k_warp_moving(FusedArgs a, int grid3, int src3, float* out) {
         for (int k = 0; k < 12; ++k) m[k] = mp[k];
         const float lx = fmaf((float)x, v.sx, v.ox), ly = fmaf((float)y, v.sy, v.oy);
         const float lz = grid3 ? fmaf((float)z, v.sz, v.oz) : v.oz;
-        out[i] = src3 ? sample3(a.src, a.src_f16, a.s, m, lx, ly, lz)
-                      : sample2(a.src, a.src_f16, a.s, m, lx, ly);
+        out[i] = src3 ? sample3(src, a.s, m, lx, ly, lz) : sample2(src, a.s, m, lx, ly);
+    }
+}
+
+// Static ultrasound patch statistics (weight half of cu_lc2_3d.cu:181-287 /
+// cu_lc2_2d.cu:88-180) per voxel in FP64: count of U > 1e-6, sum, sum of squares in a
+// fixed x-fastest order; mean and centred energy, or -1 where w = 0.
+__global__ void k_ustats(const float* img, VGeom v, int grid3, int r, double* mean, double* cyy) {
+    const long long n = (long long)v.X * v.Y * v.Z;
+    const int w = 2 * r + 1;
+    const int N = grid3 ? w * w * w : w * w;
+    const int rz = grid3 ? r : 0;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int x = (int)(i % v.X), y = (int)((i / v.X) % v.Y), z = (int)(i / ((long long)v.X * v.Y));
+        double m = 0.0, c = -1.0;
+        const bool inside = x >= r && y >= r && x + r < v.X && y + r < v.Y &&
+                            (!grid3 || (z >= r && z + r < v.Z));
+        if (inside) {
+            int cnt = 0;
+            double s1 = 0.0, s2 = 0.0;
+            for (int dz = -rz; dz <= rz; ++dz)
+                for (int dy = -r; dy <= r; ++dy) {
+                    const float* row = img + ((long long)(z + dz) * v.Y + (y + dy)) * v.X + x;
+                    for (int dx = -r; dx <= r; ++dx) {
+                        const double u = row[dx];
+                        cnt += (u > kZeroThreshold) ? 1 : 0;
+                        s1 += u;
+                        s2 += u * u;
+                    }
+                }
+            if (cnt == N) {
+                const double cc = s2 - s1 * s1 * (1.0 / (double)N);
+                if (cc * (1.0 / (double)(N - 1)) > kZeroThreshold) {
+                    m = s1 * (1.0 / (double)N);
+                    c = cc;
+                }
+            }
+        }
+        mean[i] = m;
+        cyy[i] = c;
     }
 }
 
@@ -692,31 +913,49 @@ __global__ void k_convert(const void* in, int in_dtype, void* out, int out_f16, 
 }
 
 size_t lds_bytes_2d(int r) {
+    // sized for the 9-channel (moving ultrasound) variant, the larger of the two
     const size_t PX = kTile + 2 * r, NP = PX * PX, IXE = PX + 2;
     return sizeof(double) * (NCH * NP + NCH * kTile * PX + 8) +
            sizeof(float) * (IXE * IXE + 2 * PX * IXE) + sizeof(int) * (NP + kTile * PX);
 }
 
-template <int R>
+template <int R, typename T>
 hipError_t launch3(const FusedArgs& a, bool movi, dim3 grid, hipStream_t st) {
-    if (movi) hipLaunchKernelGGL((k_fused3d<R, true>), grid, dim3(kThreads), 0, st, a);
-    else hipLaunchKernelGGL((k_fused3d<R, false>), grid, dim3(kThreads), 0, st, a);
+    if (movi) hipLaunchKernelGGL((k_fused3d<R, true, T, kTile3X, kTile3Y>), grid, dim3(kTile3X * kTile3Y), 0, st, a);
+    else hipLaunchKernelGGL((k_fused3d<R, false, T, kTile3X, kTile3Y>), grid, dim3(kTile3X * kTile3Y), 0, st, a);
     return hipGetLastError();
 }
 
-template <bool MOVI, bool SRC3D>
+template <typename T>
+hipError_t launch3r(const FusedArgs& a, bool movi, dim3 grid, hipStream_t st) {
+    switch (a.r) {
+        case 1: return launch3<1, T>(a, movi, grid, st);
+        case 2: return launch3<2, T>(a, movi, grid, st);
+        case 3: return launch3<3, T>(a, movi, grid, st);
+        case 4: return launch3<4, T>(a, movi, grid, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <bool MOVI, bool SRC3D, typename T>
 hipError_t launch2(const FusedArgs& a, dim3 grid, hipStream_t st) {
     const size_t lds = lds_bytes_2d(a.r);
     static bool configured = false;  // allow the largest radius once; smaller ones fit under it
     if (!configured) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused2d<MOVI, SRC3D>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused2d<MOVI, SRC3D, T>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)lds_bytes_2d(kMaxR2));
         if (e != hipSuccess) return e;
         configured = true;
     }
-    hipLaunchKernelGGL((k_fused2d<MOVI, SRC3D>), grid, dim3(kThreads), lds, st, a);
+    hipLaunchKernelGGL((k_fused2d<MOVI, SRC3D, T>), grid, dim3(kThreads), lds, st, a);
     return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch2t(const FusedArgs& a, bool movi, int src_ndim, dim3 grid, hipStream_t st) {
+    if (movi) return src_ndim == 3 ? launch2<true, true, T>(a, grid, st) : launch2<true, false, T>(a, grid, st);
+    return src_ndim == 3 ? launch2<false, true, T>(a, grid, st) : launch2<false, false, T>(a, grid, st);
 }
 
 }  // namespace
@@ -724,8 +963,8 @@ hipError_t launch2(const FusedArgs& a, dim3 grid, hipStream_t st) {
 int fused_blocks_3d(const VGeom& v, int r, int zchunk, int* ntx, int* nty, int* nzc) {
     const int ex = v.X - 2 * r, ey = v.Y - 2 * r, ez = v.Z - 2 * r;
     if (ex <= 0 || ey <= 0 || ez <= 0 || zchunk <= 0) { *ntx = *nty = *nzc = 0; return 0; }
-    *ntx = (ex + kTile - 1) / kTile;
-    *nty = (ey + kTile - 1) / kTile;
+    *ntx = (ex + kTile3X - 1) / kTile3X;
+    *nty = (ey + kTile3Y - 1) / kTile3Y;
     *nzc = (ez + zchunk - 1) / zchunk;
     return *ntx * *nty * *nzc;
 }
@@ -743,18 +982,12 @@ hipError_t launch_fused(const FusedArgs& a, int grid_ndim, int src_ndim, int mov
     if (a.blocks_per_pose == 0) return hipSuccess;
     if (grid_ndim == 3) {
         dim3 grid(a.ntx * a.nty, a.nzc, n_poses);
-        switch (a.r) {
-            case 1: return launch3<1>(a, movi, grid, st);
-            case 2: return launch3<2>(a, movi, grid, st);
-            case 3: return launch3<3>(a, movi, grid, st);
-            case 4: return launch3<4>(a, movi, grid, st);
-            default: return hipErrorInvalidValue;
-        }
+        return a.src_f16 ? launch3r<__half>(a, movi, grid, st) : launch3r<float>(a, movi, grid, st);
     }
     if (a.r < 1 || a.r > kMaxR2) return hipErrorInvalidValue;
     dim3 grid(a.ntx * a.nty, a.v.Z, n_poses);
-    if (movi) return src_ndim == 3 ? launch2<true, true>(a, grid, st) : launch2<true, false>(a, grid, st);
-    return src_ndim == 3 ? launch2<false, true>(a, grid, st) : launch2<false, false>(a, grid, st);
+    return a.src_f16 ? launch2t<__half>(a, movi, src_ndim, grid, st)
+                     : launch2t<float>(a, movi, src_ndim, grid, st);
 }
 
 hipError_t launch_reduce(const double* partials, int blocks_per_pose, int n_poses, double* out,
@@ -777,16 +1010,32 @@ hipError_t launch_warp_static(const void* fixed, int fixed_f16, const SGeom& fg,
     // Z = 1, which equals the bilinear per-frame path bit for bit.
     const int mode = (grid_ndim == 3) ? 0 : 1;
     (void)src_ndim;
-    hipLaunchKernelGGL(k_warp_static, grid_for(n), dim3(256), 0, st, fixed, fixed_f16, fg, v, mode,
-                       out);
+    if (fixed_f16)
+        hipLaunchKernelGGL(k_warp_static<__half>, grid_for(n), dim3(256), 0, st,
+                           static_cast<const __half*>(fixed), fg, v, mode, out);
+    else
+        hipLaunchKernelGGL(k_warp_static<float>, grid_for(n), dim3(256), 0, st,
+                           static_cast<const float*>(fixed), fg, v, mode, out);
     return hipGetLastError();
 }
 
 hipError_t launch_warp_moving(const FusedArgs& a, int grid_ndim, int src_ndim, float* out,
                               hipStream_t st) {
     const long long n = (long long)a.v.X * a.v.Y * a.v.Z;
-    hipLaunchKernelGGL(k_warp_moving, grid_for(n), dim3(256), 0, st, a, grid_ndim == 3 ? 1 : 0,
-                       src_ndim == 3 ? 1 : 0, out);
+    if (a.src_f16)
+        hipLaunchKernelGGL(k_warp_moving<__half>, grid_for(n), dim3(256), 0, st, a,
+                           grid_ndim == 3 ? 1 : 0, src_ndim == 3 ? 1 : 0, out);
+    else
+        hipLaunchKernelGGL(k_warp_moving<float>, grid_for(n), dim3(256), 0, st, a,
+                           grid_ndim == 3 ? 1 : 0, src_ndim == 3 ? 1 : 0, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_ustats(const float* img, const VGeom& v, int grid_ndim, int r, double* mean,
+                         double* cyy, hipStream_t st) {
+    const long long n = (long long)v.X * v.Y * v.Z;
+    hipLaunchKernelGGL(k_ustats, grid_for(n), dim3(256), 0, st, img, v, grid_ndim == 3 ? 1 : 0, r,
+                       mean, cyy);
     return hipGetLastError();
 }
 

@@ -4,10 +4,16 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 namespace lc2e {
 
-constexpr int kTile = 16;      // output tile edge (x and y) of the fused kernels
-constexpr int kThreads = 256;  // work-group size of the fused kernels (16 x 16 outputs)
+constexpr int kTile = 16;      // output tile edge (x and y) of the 2D fused kernel
+constexpr int kThreads = 256;  // work-group size of the 2D fused kernel (16 x 16 outputs)
+constexpr int kTile3X = 32;    // output tile of the 3D fused kernel: 32 x 16 columns,
+constexpr int kTile3Y = 16;    // one work-group of 512 threads, one column per thread
+// Resident volumes carry this many zeroed bytes past their last voxel (paired x loads).
+constexpr size_t kVolumePad = 64;
 constexpr int kMaxR3 = 4;      // largest patch radius of the 3D fused kernel
 constexpr int kMaxR2 = 10;     // largest patch radius of the 2D fused kernel
 
@@ -19,7 +25,22 @@ struct VGeom {  // virtual grid
 struct SGeom {  // a resident image
     int X, Y, Z;
     float sx, sy, sz, ox, oy, oz;
+    // 1/spacing; used only when every spacing is a power of two, where (p - o) * (1/s)
+    // equals (p - o) / s exactly
+    float rsx, rsy, rsz;
+    int pow2;
 };
+
+inline SGeom make_sgeom(const int d[3], const float s[3], const float o[3]) {
+    SGeom g{d[0], d[1], d[2], s[0], s[1], s[2], o[0], o[1], o[2], 1.0f / s[0], 1.0f / s[1],
+            1.0f / s[2], 1};
+    for (int a = 0; a < 3; ++a) {
+        int e;
+        const float m = std::frexp(s[a], &e);
+        if (m != 0.5f) g.pow2 = 0;
+    }
+    return g;
+}
 
 struct FusedArgs {
     VGeom v;
@@ -27,6 +48,8 @@ struct FusedArgs {
     const void* src;         // moving image data (float or half)
     const float* stat_img;   // fixed image resampled on the virtual grid
     const float* stat_grad;  // |Sobel| of stat_img when it is the high-resolution image
+    const double* ustat_mean;  // static ultrasound: patch mean per voxel
+    const double* ustat_cyy;   // static ultrasound: centred patch energy, < 0 where w = 0
     const float* mats;       // [pose][mats_per_pose][12] VirtualToMoving, 3x4 row-major
     double* partials;        // [pose][blocks_per_pose][2] (sum measure, sum w)
     double* wmap;            // optional per-voxel w (single pose), may be null
@@ -53,6 +76,10 @@ hipError_t launch_warp_static(const void* fixed, int fixed_f16, const SGeom& fg,
 // Moving image -> virtual grid at one pose (debug stages).
 hipError_t launch_warp_moving(const FusedArgs& a, int grid_ndim, int src_ndim, float* out,
                               hipStream_t st);
+// Patch statistics of a static ultrasound grid: mean and centred energy per voxel
+// (negative where the reference's weight is 0), FP64, fixed summation order.
+hipError_t launch_ustats(const float* img, const VGeom& v, int grid_ndim, int r, double* mean,
+                         double* cyy, hipStream_t st);
 // |Sobel| of a virtual-grid image (3D, or per frame for 2D grids), separable order.
 hipError_t launch_sobel(const float* img, const VGeom& v, int grid_ndim, float* out,
                         hipStream_t st);

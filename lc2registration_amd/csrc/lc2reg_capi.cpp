@@ -1,0 +1,146 @@
+// C ABI of the registration layer (include/lc2reg.h).
+#include "lc2reg.h"
+
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <string>
+
+#include "lc2_registration.hpp"
+#include "optim.hpp"
+#include "yaml_lite.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::exception& e) {
+    g_err = e.what();
+    return -1;
+}
+
+template <class Interp, class Opt>
+int run(const char* yaml, int write_images, int nparams, int mdim, double* out_params,
+        double* out_lc2, int* out_nevals, float* out_matrix) {
+    Interp interp{std::string(yaml)};
+    interp.InitiateMappers();
+    interp.InitiateRegistrationHandle();
+    Opt opt{std::string(yaml)};
+    opt.LinkToInterpolator(&interp);
+    opt.Optimize();
+    opt.GetOptimalTransform();
+    opt.ExportResults();
+    std::vector<float> m(mdim * mdim);
+    opt.GetInternalMatrix(m.data());
+    if (write_images) interp.WriteOut(m.data());
+    if (out_params)
+        for (int i = 0; i < nparams; ++i) out_params[i] = opt.GetOptimalSolution()[i];
+    if (out_lc2) *out_lc2 = opt.GetLastSimilarity();
+    if (out_nevals) *out_nevals = opt.GetEvaluationCount();
+    if (out_matrix) std::memcpy(out_matrix, m.data(), m.size() * sizeof(float));
+    return 0;
+}
+
+lc2opt::BatchObjective wrap(lc2reg_batch_fn f, void* user) {
+    return [f, user](int n, int d, const double* X, double* F) { f(n, d, X, F, user); };
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lc2reg_last_error(void) { return g_err.c_str(); }
+
+int lc2reg_register_yaml(const char* mode, const char* yaml_path, int write_images,
+                         double* out_params, double* out_lc2, int* out_nevals, float* out_matrix) {
+    try {
+        const std::string m = mode ? mode : "";
+        if (m == "3D")
+            return run<Interpolator3D, Optimizers3D>(yaml_path, write_images, 12, 4, out_params,
+                                                     out_lc2, out_nevals, out_matrix);
+        if (m == "2D")
+            return run<Interpolator2D, Optimizers2D>(yaml_path, write_images, 6, 3, out_params,
+                                                     out_lc2, out_nevals, out_matrix);
+        if (m == "2DTo3D")
+            return run<Interpolator2DTo3D, Optimizers2DTo3D>(yaml_path, write_images, 12, 4,
+                                                             out_params, out_lc2, out_nevals,
+                                                             out_matrix);
+        g_err = "mode must be 2D, 3D or 2DTo3D";
+        return -1;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int lc2reg_evaluate_yaml(const char* mode, const char* yaml_path, const double* params, int n,
+                         double* out_lc2) {
+    try {
+        const std::string m = mode ? mode : "";
+        std::unique_ptr<lc2reg::InterpolatorBase> ip;
+        if (m == "3D") ip.reset(new Interpolator3D(std::string(yaml_path)));
+        else if (m == "2D") ip.reset(new Interpolator2D(std::string(yaml_path)));
+        else if (m == "2DTo3D") ip.reset(new Interpolator2DTo3D(std::string(yaml_path)));
+        else {
+            g_err = "mode must be 2D, 3D or 2DTo3D";
+            return -1;
+        }
+        ip->InitiateMappers();
+        ip->InitiateRegistrationHandle();
+        std::vector<double> l;
+        ip->EvaluateBatch({std::vector<double>(params, params + n)}, l);
+        if (out_lc2) *out_lc2 = l[0];
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int lc2reg_minimize_direct(int d, const double* lb, const double* ub, int maxeval,
+                           lc2reg_batch_fn f, void* user, double* x_out, double* f_out,
+                           int* nevals) {
+    try {
+        lc2opt::Result r = lc2opt::direct_l(wrap(f, user), std::vector<double>(lb, lb + d),
+                                            std::vector<double>(ub, ub + d), maxeval);
+        if (r.code < 0) {
+            g_err = r.message;
+            return -1;
+        }
+        for (int i = 0; i < d; ++i) x_out[i] = r.x[i];
+        if (f_out) *f_out = r.f;
+        if (nevals) *nevals = r.nevals;
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int lc2reg_minimize_qmtr(int d, const double* x0, const double* lb, const double* ub,
+                         double xtol_rel, int maxeval, lc2reg_batch_fn f, void* user,
+                         double* x_out, double* f_out, int* nevals) {
+    try {
+        lc2opt::Result r = lc2opt::qmtr(wrap(f, user), std::vector<double>(x0, x0 + d),
+                                        std::vector<double>(lb, lb + d),
+                                        std::vector<double>(ub, ub + d), xtol_rel, maxeval);
+        if (r.code < 0) {
+            g_err = r.message;
+            return -1;
+        }
+        for (int i = 0; i < d; ++i) x_out[i] = r.x[i];
+        if (f_out) *f_out = r.f;
+        if (nevals) *nevals = r.nevals;
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int lc2reg_yaml_roundtrip(const char* in_path, const char* out_path) {
+    try {
+        lc2yaml::save_file(lc2yaml::load_file(in_path), out_path);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+}  // extern "C"

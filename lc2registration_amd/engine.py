@@ -76,6 +76,8 @@ def lib():
         L.lc2e_pair_set_tuning.argtypes = [vp, C.c_int]
         L.lc2e_pair_get_tuning.argtypes = [vp, C.POINTER(C.c_int)]
         L.lc2e_pair_use_graph.argtypes = [vp, C.c_int]
+        L.lc2e_pair_autotune.argtypes = [vp, C.c_int, fp, C.c_int, C.POINTER(C.c_int),
+                                         C.POINTER(C.c_double)]
         L.lc2e_pair_profile.argtypes = [vp, C.c_int]
         L.lc2e_pair_profile_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
         _lib = L
@@ -236,6 +238,15 @@ class Pair:
         ms, n = C.c_double(), C.c_longlong()
         _check(lib().lc2e_pair_profile_read(self.h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def autotune(self, mats, iters: int = 3):
+        """Pick the fastest z-chunk for batches shaped like `mats`; returns (zchunk, ms)."""
+        m = np.ascontiguousarray(mats, np.float32)
+        per = 12 * (self.info.n_frames if self.info.per_frame_matrices else 1)
+        z, ms = C.c_int(), C.c_double()
+        _check(lib().lc2e_pair_autotune(self.h, m.size // per, _fptr(m.ravel()), iters,
+                                        C.byref(z), C.byref(ms)))
+        return z.value, ms.value
 
     def use_graph(self, on: bool):
         _check(lib().lc2e_pair_use_graph(self.h, int(on)))
