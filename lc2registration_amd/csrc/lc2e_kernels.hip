@@ -15,7 +15,7 @@
 // FP32 values are exact in FP64), which removes the cancellation an FP32
 // sliding-window sum would suffer on intensities with small local variance.
 //
-// 3D structure (k_fused3d): a 256-thread work-group owns a 16x16 column of
+// 3D structure (k_fused3d): a 512-thread work-group owns a 32x16 column of
 // output voxels over `zchunk` planes and marches along z. Per input plane:
 // warp the plane (+ Sobel halo) into LDS, separable Sobel row pass in LDS,
 // column pass in registers (3-plane ring per position), FP64 products into LDS,
@@ -25,6 +25,7 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "lc2e_kernels.hpp"
@@ -311,8 +312,15 @@ __device__ __forceinline__ void block_reduce_store(double am, double aw, double*
     }
 }
 
-// Products of one position into the channel planes of the LDS buffer.
-// NC = 9 (+count): I, G, U, II, GG, IG, UU, UI, UG.  NC = 7: I, G, II, GG, IG, UI, UG.
+// LDS row strides of the products planes and of the x-pass output: an odd number of
+// doubles, so that the 16 (or 32) lanes of one LDS access that walk down consecutive rows
+// fall into distinct bank pairs (an even stride such as 32 or 38 doubles puts every row
+// into the same few banks).
+__host__ __device__ constexpr int odd_stride(int n) { return n | 1; }
+
+// Products of one position into the channel planes of the LDS buffer (plane size np,
+// position i). NC = 9 (+count): I, G, U, II, GG, IG, UU, UI, UG.
+// NC = 7: I, G, II, GG, IG, UI, UG.
 template <int NC>
 __device__ __forceinline__ void store_products(double* s_prod, int* s_cnt, int np, int i, float I,
                                                float G, float U) {
@@ -340,9 +348,11 @@ __device__ __forceinline__ void store_products(double* s_prod, int* s_cnt, int n
 }
 
 // Sliding box sums of width w along x then y, for NC FP64 channels and (CNT) one integer
-// count channel, over a px x py region into TX x TY outputs:
-//   s_prod[c][py][px] -> s_xp[c][py][TX] -> s_out[c][TY][TX]   (s_out may alias s_prod).
-// Rows (x pass) and columns (y pass) are cut into runs of 16 outputs, one task each.
+// count channel, over a region of py rows (row stride ps >= TX + w - 1) into TX x TY
+// outputs:
+//   s_prod[c][py][ps] -> s_xp[c][py][TX + 1] -> s_out[c][TY][TX]   (s_out may alias s_prod).
+// Rows (x pass) and columns (y pass) are cut into runs of 16 outputs, one task each;
+// neighbouring lanes take neighbouring rows (x pass) or columns (y pass).
 template <typename V>
 __device__ __forceinline__ void slide16(const V* in, int stride, int w, V* out, int ostride) {
     V acc = 0;
@@ -357,30 +367,31 @@ __device__ __forceinline__ void slide16(const V* in, int stride, int w, V* out, 
 
 template <int NC, bool CNT, int TX, int TY, int NTHR>
 __device__ __forceinline__ void box_xy(const double* s_prod, const int* s_cnt, double* s_xp,
-                                       int* s_xcnt, double* s_out, int* s_ocnt, int px, int py,
+                                       int* s_xcnt, double* s_out, int* s_ocnt, int ps, int py,
                                        int w, int tid) {
     static_assert(TX % 16 == 0 && TY % 16 == 0, "tile edges are multiples of 16");
     constexpr int NT = NC + (CNT ? 1 : 0);
     constexpr int SX = TX / 16, SY = TY / 16;
-    const int np = px * py;
+    constexpr int XS = odd_stride(TX + 1);
+    const int np = ps * py;
     for (int t = tid; t < NT * py * SX; t += NTHR) {
-        const int seg = t % SX, rest = t / SX;
-        const int row = rest % py, c = rest / py;
+        const int row = t % py, rest = t / py;
+        const int seg = rest % SX, c = rest / SX;
         if (c < NC)
-            slide16(s_prod + c * np + row * px + seg * 16, 1, w,
-                    s_xp + (c * py + row) * TX + seg * 16, 1);
+            slide16(s_prod + c * np + row * ps + seg * 16, 1, w,
+                    s_xp + (c * py + row) * XS + seg * 16, 1);
         else
-            slide16(s_cnt + row * px + seg * 16, 1, w, s_xcnt + row * TX + seg * 16, 1);
+            slide16(s_cnt + row * ps + seg * 16, 1, w, s_xcnt + row * XS + seg * 16, 1);
     }
     __syncthreads();
     for (int t = tid; t < NT * TX * SY; t += NTHR) {
-        const int seg = t % SY, rest = t / SY;
-        const int col = rest % TX, c = rest / TX;
+        const int col = t % TX, rest = t / TX;
+        const int seg = rest % SY, c = rest / SY;
         if (c < NC)
-            slide16(s_xp + c * py * TX + seg * 16 * TX + col, TX, w,
+            slide16(s_xp + (c * py + seg * 16) * XS + col, XS, w,
                     s_out + c * TX * TY + seg * 16 * TX + col, TX);
         else
-            slide16(s_xcnt + seg * 16 * TX + col, TX, w, s_ocnt + seg * 16 * TX + col, TX);
+            slide16(s_xcnt + seg * 16 * XS + col, XS, w, s_ocnt + seg * 16 * TX + col, TX);
     }
     __syncthreads();
 }
@@ -400,6 +411,8 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
     constexpr int W = 2 * R + 1;
     constexpr int PX = TX + 2 * R, PY = TY + 2 * R;  // patch-support region
     constexpr int NP = PX * PY;
+    constexpr int PS = odd_stride(PX);               // LDS row stride of the products planes
+    constexpr int XS = odd_stride(TX + 1);           // LDS row stride of the x-pass output
     constexpr int IXE = PX + 2, IYE = PY + 2;        // Sobel input region
     constexpr int NI = MOVI ? IXE * IYE : NP;        // warped positions per plane
     constexpr int SPT = (NI + NTHR - 1) / NTHR;      // warped samples per thread
@@ -412,10 +425,10 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
     __shared__ float s_in[NI];
     __shared__ float s_s[MOVI ? PX * IYE : 1];
     __shared__ float s_d[MOVI ? PX * IYE : 1];
-    __shared__ double s_prod[NC * NP];
-    __shared__ int s_cnt[CNT ? NP : 1];
-    __shared__ double s_xp[NC * TX * PY];
-    __shared__ int s_xcnt[CNT ? TX * PY : 1];
+    __shared__ double s_prod[NC * PY * PS];
+    __shared__ int s_cnt[CNT ? PY * PS : 1];
+    __shared__ double s_xp[NC * PY * XS];
+    __shared__ int s_xcnt[CNT ? PY * XS : 1];
     __shared__ double s_red[2 * NTHR / 64];
 
     const int tid = threadIdx.x;
@@ -576,14 +589,14 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
                             I = cur0[j];
                             G = cur1[j];
                         }
-                        store_products<NC>(s_prod, s_cnt, NP, i, I, G, U);
+                        store_products<NC>(s_prod, s_cnt, PY * PS, ly * PS + lx, I, G, U);
                     }
                 }
             }
             __syncthreads();  // s_in / s_s / s_d reads done; products visible
             if (q < 0) return;  // block-uniform: Sobel ring warm-up only
             // 4. in-plane box sums
-            box_xy<NC, CNT, TX, TY, NTHR>(s_prod, s_cnt, s_xp, s_xcnt, s_prod, s_cnt, PX, PY, W,
+            box_xy<NC, CNT, TX, TY, NTHR>(s_prod, s_cnt, s_xp, s_xcnt, s_prod, s_cnt, PS, PY, W,
                                           tid);
             // 5. z ring of plane sums, running patch sums, combine
             constexpr int slot = (k - LAG + W) % W;
@@ -634,15 +647,20 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
     constexpr int NC = MOVI ? 7 : 9;
     constexpr bool CNT = !MOVI;
     const int R = a.r, W = 2 * R + 1, PX = kTile + 2 * R, NP = PX * PX, IXE = PX + 2;
+    const int PS = odd_stride(PX);                            // products row stride
+    constexpr int XS = odd_stride(kTile + 1);                 // x-pass row stride
     const int NI = MOVI ? IXE * IXE : NP;
-    double* s_prod = s_dyn;                                   // NC*NP
-    double* s_xp = s_prod + NC * NP;                          // NC*kTile*PX
-    double* s_red = s_xp + NC * kTile * PX;                   // 8
-    float* s_in = reinterpret_cast<float*>(s_red + 8);        // NI
+    // The x-pass output s_xp shares its space with the warp / Sobel scratch (s_in, s_s,
+    // s_d): those are dead once the products are stored, before the box sums start.
+    const int uni = max(2 * NC * PX * XS, NI + 2 * PX * IXE);  // shared region, in floats
+    double* s_prod = s_dyn;                                   // NC*PX*PS
+    double* s_red = s_prod + NC * PX * PS;                    // 8
+    double* s_xp = s_red + 8;                                 // NC*PX*XS
+    float* s_in = reinterpret_cast<float*>(s_xp);             // NI
     float* s_s = s_in + NI;                                   // PX*IXE
     float* s_d = s_s + PX * IXE;                              // PX*IXE
-    int* s_cnt = reinterpret_cast<int*>(s_d + PX * IXE);      // NP
-    int* s_xcnt = s_cnt + NP;                                 // kTile*PX
+    int* s_cnt = reinterpret_cast<int*>(s_in + uni);          // PX*PS
+    int* s_xcnt = s_cnt + PX * PS;                            // PX*XS
 
     const int tid = threadIdx.x;
     const int tx = blockIdx.x % a.ntx, ty = blockIdx.x / a.ntx;
@@ -715,10 +733,10 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
             I = in ? a.stat_img[vidx] : 0.0f;
             G = in ? a.stat_grad[vidx] : 0.0f;
         }
-        store_products<NC>(s_prod, s_cnt, NP, i, I, G, U);
+        store_products<NC>(s_prod, s_cnt, PX * PS, ly * PS + lx, I, G, U);
     }
     __syncthreads();
-    box_xy<NC, CNT, kTile, kTile, kThreads>(s_prod, s_cnt, s_xp, s_xcnt, s_prod, s_cnt, PX, PX, W,
+    box_xy<NC, CNT, kTile, kTile, kThreads>(s_prod, s_cnt, s_xp, s_xcnt, s_prod, s_cnt, PS, PX, W,
                                             tid);
     const int ox = tid % kTile, oy = tid / kTile;
     double acc_m = 0.0, acc_w = 0.0;
@@ -912,12 +930,15 @@ __global__ void k_convert(const void* in, int in_dtype, void* out, int out_f16, 
     }
 }
 
-size_t lds_bytes_2d(int r) {
-    // sized for the 9-channel (moving ultrasound) variant, the larger of the two
-    const size_t PX = kTile + 2 * r, NP = PX * PX, IXE = PX + 2;
-    return sizeof(double) * (NCH * NP + NCH * kTile * PX + 8) +
-           sizeof(float) * (IXE * IXE + 2 * PX * IXE) + sizeof(int) * (NP + kTile * PX);
+constexpr size_t lds_bytes_2d(int r) {
+    // the layout of k_fused2d for the 9-channel (moving ultrasound) variant, the larger one
+    const size_t PX = kTile + 2 * r, IXE = PX + 2;
+    const size_t PS = odd_stride((int)PX), XS = odd_stride(kTile + 1);
+    const size_t uni = std::max(2 * NCH * PX * XS, IXE * IXE + 2 * PX * IXE);
+    return sizeof(double) * (NCH * PX * PS + 8) + sizeof(float) * uni +
+           sizeof(int) * (PX * PS + PX * XS);
 }
+static_assert(lds_bytes_2d(kMaxR2) <= 160 * 1024, "2D fused kernel exceeds the 160 KiB of LDS");
 
 template <int R, typename T>
 hipError_t launch3(const FusedArgs& a, bool movi, dim3 grid, hipStream_t st) {
