@@ -28,7 +28,7 @@ FP64_PEAK = 78.6e12
 ALG_BYTES_PER_VOXEL = 1.0 + 0.5
 ALG_F32_PER_VOXEL = 116.0   # warp 40, Sobel 27, products 5, window sums 42, reduction 2
 ALG_F64_PER_VOXEL = 45.0    # centred moments, 2x2 solve, variance/measure, thresholds
-PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "fused3d_cfg1_traffic.json")
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "k_fused3d_r3_affine3d_us192_traffic.json")
 
 
 def log(msg):
@@ -75,6 +75,18 @@ def cpu_baseline(w, M, budget_s):
             "exact_fp64_evals_per_s": out["exact_fp64"][0]}
 
 
+def calibrate_copy(nbytes):
+    """A copy kernel of known size: under rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per
+    dispatch, divided by nbytes, give the factor that turns those counters into bytes."""
+    import torch
+    a = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "dispatches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,7 +97,13 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds per CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip single-pose / fp16 figures")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device-to-device copy of BYTES three times (the "
+                         "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1",
                           "n_gpus": args.gpus}))
