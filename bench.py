@@ -126,7 +126,7 @@ def main():
     mats = np.stack([pair.pose_to_matrix(p) for p in poses])
     P = len(mats)
     if args.zchunk:
-        pair.set_tuning(args.zchunk)
+        pair.set_tuning_n(P, args.zchunk)
     else:
         from lc2registration_amd import autotune
         zc = autotune.tune(pair, mats)
@@ -148,6 +148,7 @@ def main():
     elapsed = time.perf_counter() - t_start
     kernel_ms, launches = pair.profile_read()
     pair.profile(False)
+    zchunk_batch = pair.get_tuning()
     avg_kernel_s = kernel_ms / max(launches, 1) / 1e3
     value = args.steps * P / elapsed
     log(f"{value:.1f} evals/s, kernel {avg_kernel_s * 1e3:.3f} ms per launch of {P} poses")
@@ -214,7 +215,7 @@ def main():
         "config": {"workload": "3D affine LC2: MR 256x256x192 f32 (1 mm, moving) vs US "
                                "192x192x160 u8 (0.5 mm, fixed grid), patch 7^3",
                    "poses_per_step": P, "virtual_grid": list(pair.virt_dims),
-                   "zchunk": pair.get_tuning()},
+                   "zchunk": zchunk_batch},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "hbm_gbs_algorithmic": achieved,

@@ -137,12 +137,20 @@ int lc2e_sync(lc2e_engine* e);
  * (WARP_I, WARP_US, GRAD) or float64 (W, MEASURE). */
 int lc2e_debug_fetch(lc2e_pair* p, lc2e_stage stage, const float* m12, void* host);
 
-/* Launch configuration of the fused kernel (z planes per work-group; 0 = heuristic). */
+/* Launch configuration of the 3D fused kernel: output planes per work-group (z-chunk).
+ * Each launch of n poses uses, in order: the value forced by lc2e_pair_set_tuning
+ * (0 = none), the value set or tuned for batches of exactly n poses
+ * (lc2e_pair_set_tuning_n, lc2e_pair_autotune; 0 = none), or a model of the launch
+ * (work-group rounds over the CUs times the planes each work-group marches).
+ * get_tuning reports the forced value, else the one the last launch used.
+ * No counterpart in the reference (its launch shape is fixed at 8x8x8). */
 int lc2e_pair_set_tuning(lc2e_pair* p, int zchunk);
+int lc2e_pair_set_tuning_n(lc2e_pair* p, int n, int zchunk);
 int lc2e_pair_get_tuning(lc2e_pair* p, int* zchunk);
 /* Autotuner: times the fused kernel over candidate z-chunks (HIP events, `iters` launches
- * each after one warm-up) for batches of n poses `mats`, keeps the fastest and returns it.
- * 2D pairs have nothing to tune and return 1. No counterpart in the reference. */
+ * each after one warm-up) for batches of n poses `mats`, records the fastest for batches
+ * of n (clearing a forced value) and returns it. 2D pairs have nothing to tune and
+ * return 1. No counterpart in the reference. */
 int lc2e_pair_autotune(lc2e_pair* p, int n, const float* mats, int iters, int* best_zchunk,
                        double* best_ms);
 

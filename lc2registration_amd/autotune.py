@@ -32,12 +32,12 @@ def load(path: str = DEFAULT_CACHE) -> dict:
 
 
 def tune(pair, mats, iters: int = 3, path: str = DEFAULT_CACHE, force: bool = False) -> int:
-    """Set the pair's z-chunk for batches shaped like `mats` (cached); returns it."""
+    """Set the pair's z-chunk for batches of len(mats) poses (cached); returns it."""
     n = len(mats)
     cache = load(path)
     key = _key(pair, n)
     if not force and key in cache:
-        pair.set_tuning(int(cache[key]["zchunk"]))
+        pair.set_tuning_n(n, int(cache[key]["zchunk"]))
         return int(cache[key]["zchunk"])
     z, ms = pair.autotune(mats, iters)
     cache[key] = {"zchunk": z, "ms_per_launch": ms}

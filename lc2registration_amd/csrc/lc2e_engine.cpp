@@ -50,6 +50,7 @@ int dtype_size(int dt) {
 
 struct lc2e_engine {
     int device = 0;
+    int cus = 0;  // compute units of the device
     hipStream_t stream = nullptr;
     size_t budget = 0;  // 0 = unlimited
     size_t used = 0;
@@ -95,10 +96,14 @@ struct lc2e_pair {
     float* stat_grad = nullptr;
     double* ustat = nullptr;  // [2][V]: patch mean and centred energy of a static ultrasound
     size_t vbytes = 0;
-    int zchunk = 0;
-    int ntx = 0, nty = 0, nzc = 0, blocks_per_pose = 0;
+    // 3D z-chunk: forced for every launch (lc2e_pair_set_tuning), else the value tuned for
+    // the launch's batch size (lc2e_pair_autotune / lc2e_pair_set_tuning_n), else a model
+    int zchunk_forced = 0;
+    std::map<int, int> tuned;
+    int last_zchunk = 0;
     // batch buffers
     int cap = 0;
+    int part_bpp = 0;  // partial sums per pose the partials buffer holds
     float* d_mats = nullptr;
     double* d_partials = nullptr;
     double* d_out = nullptr;
@@ -126,39 +131,64 @@ void drop_graphs(lc2e_pair* p) {
     p->graphs.clear();
 }
 
-int heuristic_zchunk(const lc2e_pair* p) {
-    // Enough work-groups per pose to cover 256 CUs x 2 resident groups for a single pose,
-    // while keeping the z halo overhead (2R+2 extra planes) small.
-    if (p->info.grid_ndim != 3) return 1;
+// Work decomposition of one launch.
+struct Tiling {
+    int zchunk = 1, ntx = 0, nty = 0, nzc = 0;
+    int tiles = 0;  // work-groups per pose
+    int bpp = 0;    // partial sums per pose
+};
+
+int round_zchunk(int c) { return (c + lc2e::kZBlock - 1) / lc2e::kZBlock * lc2e::kZBlock; }
+
+// z-chunk for n poses when nothing is tuned: the 3D kernel runs one 512-thread work-group
+// per CU, every work-group marches chunk + 2R + 2 planes, so a launch costs about
+// ceil(work-groups / CUs) rounds x (chunk + 2R + 2) plane steps. Minimise that over the
+// chunk lengths (multiples of kZBlock; ties: fewer, longer chunks).
+int model_zchunk(const lc2e_pair* p, int n) {
     const int r = p->info.patch_radius;
-    int ntx, nty, nzc;
-    const int cols = (lc2e::fused_blocks_3d(p->v, r, 1 << 20, &ntx, &nty, &nzc) > 0) ? ntx * nty : 1;
     const int ez = p->v.Z - 2 * r;
-    int zc = 64;
-    while (zc > 8 && cols * ((ez + zc - 1) / zc) < 256) zc /= 2;
-    return zc;
+    int ntx, nty, nzc;
+    if (ez <= 0 || lc2e::fused_blocks_3d(p->v, r, ez, &ntx, &nty, &nzc) <= 0) return lc2e::kZBlock;
+    const long long cols = (long long)ntx * nty;
+    const int cus = p->e->cus > 0 ? p->e->cus : 256;
+    const int cmax = round_zchunk(ez);
+    int best = cmax;
+    long long best_cost = -1;
+    for (int c = cmax; c >= lc2e::kZBlock; c -= lc2e::kZBlock) {
+        const int chunks = (ez + c - 1) / c;
+        const long long tiles8 = (cols * chunks + lc2e::kXcds - 1) / lc2e::kXcds * lc2e::kXcds;
+        const long long rounds = (tiles8 * n + cus - 1) / cus;
+        const long long cost = rounds * (c + 2 * r + 2);
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = c;
+        }
+    }
+    return best;
 }
 
-int update_tiling(lc2e_pair* p) {
+Tiling tiling_for(lc2e_pair* p, int n) {
+    Tiling t;
+    const int r = p->info.patch_radius;
     if (p->info.grid_ndim == 3) {
-        if (p->zchunk <= 0) p->zchunk = heuristic_zchunk(p);
-        p->blocks_per_pose = lc2e::fused_blocks_3d(p->v, p->info.patch_radius, p->zchunk, &p->ntx,
-                                                   &p->nty, &p->nzc);
+        auto it = p->tuned.find(n);
+        t.zchunk = round_zchunk(p->zchunk_forced > 0 ? p->zchunk_forced
+                                : it != p->tuned.end() ? it->second : model_zchunk(p, n));
+        t.tiles = lc2e::fused_blocks_3d(p->v, r, t.zchunk, &t.ntx, &t.nty, &t.nzc);
+        t.bpp = lc2e::fused_partials_3d(p->v, r);
+        p->last_zchunk = t.zchunk;
     } else {
-        p->zchunk = 1;
-        p->blocks_per_pose = lc2e::fused_blocks_2d(p->v, p->info.patch_radius, &p->ntx, &p->nty);
-        p->nzc = p->v.Z;
+        t.tiles = t.bpp = lc2e::fused_blocks_2d(p->v, r, &t.ntx, &t.nty);
+        t.nzc = p->v.Z;
     }
-    return LC2E_OK;
+    return t;
 }
 
 void free_batch(lc2e_pair* p) {
     drop_graphs(p);
     lc2e_engine* e = p->e;
     if (p->d_mats) e->release(p->d_mats, (size_t)p->cap * p->mats_per_pose() * 12 * sizeof(float));
-    if (p->d_partials)
-        e->release(p->d_partials, (size_t)p->cap * (p->blocks_per_pose > 0 ? p->blocks_per_pose : 1) *
-                                      2 * sizeof(double));
+    if (p->d_partials) e->release(p->d_partials, (size_t)p->cap * p->part_bpp * 2 * sizeof(double));
     if (p->d_out) e->release(p->d_out, (size_t)p->cap * 3 * sizeof(double));
     if (p->h_mats) (void)hipHostFree(p->h_mats);
     if (p->h_out) (void)hipHostFree(p->h_out);
@@ -168,16 +198,19 @@ void free_batch(lc2e_pair* p) {
     p->h_mats = nullptr;
     p->h_out = nullptr;
     p->cap = 0;
+    p->part_bpp = 0;
 }
 
-int ensure_capacity(lc2e_pair* p, int n) {
-    if (n <= p->cap) return LC2E_OK;
-    free_batch(p);
-    int cap = 1;
+// Batch buffers for n poses whose launches use up to bpp work-groups per pose.
+int ensure_capacity(lc2e_pair* p, int n, int bpp) {
+    if (bpp < 1) bpp = 1;
+    if (n <= p->cap && bpp <= p->part_bpp) return LC2E_OK;
+    int cap = p->cap > 1 ? p->cap : 1;
     while (cap < n) cap *= 2;
+    if (bpp < p->part_bpp) bpp = p->part_bpp;
+    free_batch(p);
     lc2e_engine* e = p->e;
     const int mpp = p->mats_per_pose();
-    const int bpp = p->blocks_per_pose > 0 ? p->blocks_per_pose : 1;
     int rc;
     if ((rc = e->alloc((void**)&p->d_mats, (size_t)cap * mpp * 12 * sizeof(float))) < 0) return rc;
     if ((rc = e->alloc((void**)&p->d_partials, (size_t)cap * bpp * 2 * sizeof(double))) < 0) {
@@ -196,10 +229,11 @@ int ensure_capacity(lc2e_pair* p, int n) {
                           hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void**)&p->h_out, (size_t)cap * 3 * sizeof(double), hipHostMallocDefault));
     p->cap = cap;
+    p->part_bpp = bpp;
     return LC2E_OK;
 }
 
-lc2e::FusedArgs make_args(lc2e_pair* p, const float* d_mats, double* d_partials) {
+lc2e::FusedArgs make_args(lc2e_pair* p, const float* d_mats, double* d_partials, const Tiling& t) {
     lc2e::FusedArgs a{};
     a.v = p->v;
     a.s = p->sg;
@@ -215,28 +249,29 @@ lc2e::FusedArgs make_args(lc2e_pair* p, const float* d_mats, double* d_partials)
     a.wmap = nullptr;
     a.mmap = nullptr;
     a.r = p->info.patch_radius;
-    a.zchunk = p->zchunk;
-    a.ntx = p->ntx;
-    a.nty = p->nty;
-    a.nzc = p->nzc;
-    a.blocks_per_pose = p->blocks_per_pose;
+    a.zchunk = t.zchunk;
+    a.ntx = t.ntx;
+    a.nty = t.nty;
+    a.nzc = t.nzc;
+    a.blocks_per_pose = t.bpp;
+    a.tiles = t.tiles;
     a.mats_per_pose = p->mats_per_pose();
     return a;
 }
 
 // Enqueue: H->D of the staged matrices, fused kernel, reduction, D->H of the results.
-int enqueue_batch(lc2e_pair* p, int n, hipStream_t st) {
+int enqueue_batch(lc2e_pair* p, int n, const Tiling& t, hipStream_t st) {
     const int mpp = p->mats_per_pose();
     HIP_TRY(hipMemcpyAsync(p->d_mats, p->h_mats, (size_t)n * mpp * 12 * sizeof(float),
                            hipMemcpyHostToDevice, st));
-    if (p->blocks_per_pose > 0) {
-        lc2e::FusedArgs a = make_args(p, p->d_mats, p->d_partials);
+    if (t.tiles > 0) {
+        lc2e::FusedArgs a = make_args(p, p->d_mats, p->d_partials, t);
         const bool timed = p->profile && !p->use_graph;
         if (timed) HIP_TRY(hipEventRecord(p->ev0, st));
         HIP_TRY(lc2e::launch_fused(a, p->info.grid_ndim, p->info.src_ndim,
                                    p->info.high_res_is_moving, n, st));
         if (timed) HIP_TRY(hipEventRecord(p->ev1, st));
-        HIP_TRY(lc2e::launch_reduce(p->d_partials, p->blocks_per_pose, n, p->d_out, st));
+        HIP_TRY(lc2e::launch_reduce(p->d_partials, t.bpp, n, p->d_out, st));
     } else {
         // no patch centre fits in the grid: every sum is zero
         HIP_TRY(lc2e::launch_reduce(p->d_partials, 0, n, p->d_out, st));
@@ -246,14 +281,15 @@ int enqueue_batch(lc2e_pair* p, int n, hipStream_t st) {
     return LC2E_OK;
 }
 
-int run_batch(lc2e_pair* p, int n) {
+// Runs a batch of n poses staged in h_mats; the buffers must already fit tiling t.
+int run_batch(lc2e_pair* p, int n, const Tiling& t) {
     hipStream_t st = p->e->stream;
     if (p->use_graph) {
         auto it = p->graphs.find(n);
         if (it == p->graphs.end()) {
             GraphEntry g;
             HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            int rc = enqueue_batch(p, n, st);
+            int rc = enqueue_batch(p, n, t, st);
             hipGraph_t graph = nullptr;
             hipError_t ce = hipStreamEndCapture(st, &graph);
             if (rc < 0) return rc;
@@ -264,11 +300,11 @@ int run_batch(lc2e_pair* p, int n) {
         }
         HIP_TRY(hipGraphLaunch(it->second.exec, st));
     } else {
-        int rc = enqueue_batch(p, n, st);
+        int rc = enqueue_batch(p, n, t, st);
         if (rc < 0) return rc;
     }
     HIP_TRY(hipStreamSynchronize(st));
-    if (p->profile && !p->use_graph && p->blocks_per_pose > 0) {
+    if (p->profile && !p->use_graph && t.tiles > 0) {
         float ms = 0.0f;
         HIP_TRY(hipEventElapsedTime(&ms, p->ev0, p->ev1));
         p->prof_ms += ms;
@@ -293,6 +329,8 @@ int lc2e_create(int device, size_t hbm_budget_bytes, lc2e_engine** out) {
     e->device = device;
     e->budget = hbm_budget_bytes;
     hipError_t err = hipSetDevice(device);
+    if (err == hipSuccess)
+        err = hipDeviceGetAttribute(&e->cus, hipDeviceAttributeMultiprocessorCount, device);
     if (err == hipSuccess) err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
     if (err != hipSuccess) {
         delete e;
@@ -456,7 +494,6 @@ int lc2e_pair_create(lc2e_engine* e, lc2e_volume* fixed, lc2e_volume* moving,
         lc2e_pair_destroy(p);
         return fail(LC2E_ERR_HIP, "pair set-up failed: %s", hipGetErrorString(err));
     }
-    update_tiling(p);
     *out = p;
     return LC2E_OK;
 }
@@ -505,10 +542,11 @@ int lc2e_eval_batch(lc2e_pair* p, int n, const float* mats, double* lc2, double*
     int status = LC2E_OK;
     for (int base = 0; base < n; base += chunk_max) {
         const int m = (n - base) < chunk_max ? (n - base) : chunk_max;
-        int rc = ensure_capacity(p, m);
+        const Tiling t = tiling_for(p, m);
+        int rc = ensure_capacity(p, m, t.bpp);
         if (rc < 0) return rc;
         std::memcpy(p->h_mats, mats + (size_t)base * mpp * 12, (size_t)m * mpp * 12 * sizeof(float));
-        rc = run_batch(p, m);
+        rc = run_batch(p, m, t);
         if (rc < 0) return rc;
         for (int i = 0; i < m; ++i) {
             const double sm = p->h_out[3 * i], sw = p->h_out[3 * i + 1], l = p->h_out[3 * i + 2];
@@ -529,14 +567,15 @@ int lc2e_eval_batch_async(lc2e_pair* p, int n, const float* d_mats, double* d_ou
     if (!p || !d_mats || !d_out || n <= 0) return fail(LC2E_ERR_INVALID, "invalid argument");
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : p->e->stream;
     // partials are per pair; size them for n
-    int rc = ensure_capacity(p, n);
+    const Tiling t = tiling_for(p, n);
+    int rc = ensure_capacity(p, n, t.bpp);
     if (rc < 0) return rc;
-    if (p->blocks_per_pose > 0) {
-        lc2e::FusedArgs a = make_args(p, d_mats, p->d_partials);
+    if (t.tiles > 0) {
+        lc2e::FusedArgs a = make_args(p, d_mats, p->d_partials, t);
         HIP_TRY(lc2e::launch_fused(a, p->info.grid_ndim, p->info.src_ndim,
                                    p->info.high_res_is_moving, n, st));
     }
-    HIP_TRY(lc2e::launch_reduce(p->d_partials, p->blocks_per_pose, n, d_out, st));
+    HIP_TRY(lc2e::launch_reduce(p->d_partials, t.bpp, n, d_out, st));
     return LC2E_OK;
 }
 
@@ -552,10 +591,11 @@ int lc2e_debug_fetch(lc2e_pair* p, lc2e_stage stage, const float* m12, void* hos
     hipStream_t st = p->e->stream;
     const long long V = (long long)p->v.X * p->v.Y * p->v.Z;
     const int mpp = p->mats_per_pose();
-    int rc = ensure_capacity(p, 1);
+    const Tiling t = tiling_for(p, 1);
+    int rc = ensure_capacity(p, 1, t.bpp);
     if (rc < 0) return rc;
     HIP_TRY(hipMemcpy(p->d_mats, m12, (size_t)mpp * 12 * sizeof(float), hipMemcpyHostToDevice));
-    lc2e::FusedArgs a = make_args(p, p->d_mats, p->d_partials);
+    lc2e::FusedArgs a = make_args(p, p->d_mats, p->d_partials, t);
     const bool movi = p->info.high_res_is_moving != 0;
     if (stage == LC2E_STAGE_WARP_I || stage == LC2E_STAGE_WARP_US || stage == LC2E_STAGE_GRAD) {
         const bool want_moving = (stage == LC2E_STAGE_WARP_I) == movi || (stage == LC2E_STAGE_GRAD && movi);
@@ -589,7 +629,7 @@ int lc2e_debug_fetch(lc2e_pair* p, lc2e_stage stage, const float* m12, void* hos
     hipError_t err = hipMemsetAsync(maps, 0, 2 * V * sizeof(double), st);
     a.wmap = maps;
     a.mmap = maps + V;
-    if (err == hipSuccess && p->blocks_per_pose > 0)
+    if (err == hipSuccess && t.tiles > 0)
         err = lc2e::launch_fused(a, p->info.grid_ndim, p->info.src_ndim, p->info.high_res_is_moving, 1, st);
     if (err == hipSuccess)
         err = hipMemcpyAsync(host, stage == LC2E_STAGE_W ? a.wmap : a.mmap, V * sizeof(double),
@@ -603,9 +643,18 @@ int lc2e_debug_fetch(lc2e_pair* p, lc2e_stage stage, const float* m12, void* hos
 int lc2e_pair_set_tuning(lc2e_pair* p, int zchunk) {
     if (!p || zchunk < 0) return fail(LC2E_ERR_INVALID, "invalid argument");
     (void)hipStreamSynchronize(p->e->stream);
-    free_batch(p);
-    p->zchunk = zchunk;
-    return update_tiling(p);
+    drop_graphs(p);  // captured launches carry the old decomposition
+    p->zchunk_forced = zchunk > 0 ? round_zchunk(zchunk) : 0;
+    return LC2E_OK;
+}
+
+int lc2e_pair_set_tuning_n(lc2e_pair* p, int n, int zchunk) {
+    if (!p || n <= 0 || zchunk < 0) return fail(LC2E_ERR_INVALID, "invalid argument");
+    (void)hipStreamSynchronize(p->e->stream);
+    drop_graphs(p);
+    if (zchunk == 0) p->tuned.erase(n);
+    else p->tuned[n] = round_zchunk(zchunk);
+    return LC2E_OK;
 }
 
 int lc2e_pair_autotune(lc2e_pair* p, int n, const float* mats, int iters, int* best_zchunk,
@@ -617,24 +666,35 @@ int lc2e_pair_autotune(lc2e_pair* p, int n, const float* mats, int iters, int* b
         return LC2E_OK;
     }
     const int ez = p->v.Z - 2 * p->info.patch_radius;
-    static const int cand[] = {8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256};
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    // chunks shorter than the column, then the whole column in one chunk
+    std::vector<int> cand;
+    for (int zc : {8, 16, 24, 32, 48, 64, 96, 128, 192, 256})
+        if (zc < ez) cand.push_back(zc);
+    cand.push_back(ez > 0 ? round_zchunk(ez) : lc2e::kZBlock);
+    struct Events {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Events() {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        }
+    } ev;
+    HIP_TRY(hipEventCreate(&ev.e0));
+    HIP_TRY(hipEventCreate(&ev.e1));
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     int best = 0;
     double bms = 1e300;
     hipStream_t st = p->e->stream;
-    const bool graph = p->use_graph;
-    p->use_graph = false;
+    drop_graphs(p);
+    const int forced = p->zchunk_forced;
     for (int zc : cand) {
-        if (zc > ez && zc != cand[0]) break;
-        int rc = lc2e_pair_set_tuning(p, zc);
-        if (rc < 0) return rc;
-        rc = ensure_capacity(p, n);
+        p->zchunk_forced = zc;
+        const Tiling t = tiling_for(p, n);
+        p->zchunk_forced = forced;
+        int rc = ensure_capacity(p, n, t.bpp);
         if (rc < 0) return rc;
         HIP_TRY(hipMemcpy(p->d_mats, mats, (size_t)n * p->mats_per_pose() * 12 * sizeof(float),
                           hipMemcpyHostToDevice));
-        lc2e::FusedArgs a = make_args(p, p->d_mats, p->d_partials);
+        lc2e::FusedArgs a = make_args(p, p->d_mats, p->d_partials, t);
         HIP_TRY(lc2e::launch_fused(a, 3, p->info.src_ndim, p->info.high_res_is_moving, n, st));
         HIP_TRY(hipEventRecord(e0, st));
         for (int it = 0; it < iters; ++it)
@@ -648,11 +708,8 @@ int lc2e_pair_autotune(lc2e_pair* p, int n, const float* mats, int iters, int* b
             best = zc;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    p->use_graph = graph;
-    int rc = lc2e_pair_set_tuning(p, best);
-    if (rc < 0) return rc;
+    p->zchunk_forced = 0;  // the tuned value now applies to batches of n
+    p->tuned[n] = best;
     if (best_zchunk) *best_zchunk = best;
     if (best_ms) *best_ms = bms;
     return LC2E_OK;
@@ -660,7 +717,9 @@ int lc2e_pair_autotune(lc2e_pair* p, int n, const float* mats, int iters, int* b
 
 int lc2e_pair_get_tuning(lc2e_pair* p, int* zchunk) {
     if (!p || !zchunk) return fail(LC2E_ERR_INVALID, "invalid argument");
-    *zchunk = p->zchunk;
+    *zchunk = p->zchunk_forced > 0 ? p->zchunk_forced
+              : p->last_zchunk > 0 ? p->last_zchunk
+              : (p->info.grid_ndim == 3 ? round_zchunk(model_zchunk(p, 1)) : 1);
     return LC2E_OK;
 }
 

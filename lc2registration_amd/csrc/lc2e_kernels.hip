@@ -129,50 +129,72 @@ __device__ __forceinline__ float sample3(const T* __restrict__ src, const SGeom&
 
 // The same trilinear sample split in two for software pipelining: issue3() computes the
 // voxel coordinates and starts the four paired loads, finish3() does the lerps once the
-// data has arrived, so a whole z-step of work can run in between.
+// data has arrived, so a whole z-step of work can run in between. Branch-free: a sample
+// outside the source (or a position outside the grid, inside == false) still loads from
+// coordinates clamped into the volume and is zeroed in finish3(); inside the volume the
+// clamp changes nothing, so the result is the reference's bit for bit.
 struct Pending3 {
     float a0, a1, b0, b1, c0, c1, d0, d1;
     float rx, ry, rz;
-    int ok;
+    bool ok, xs;
 };
+
+template <typename T>
+__device__ __forceinline__ void ld2_off(const T* __restrict__ base, unsigned idx, float& a,
+                                        float& b) {
+    // 32-bit element offset from a uniform base: one saddr + voffset load, no 64-bit math
+    const char* p = reinterpret_cast<const char*>(base) + idx * (unsigned)sizeof(T);
+    if constexpr (sizeof(T) == 2) {
+        __half2 h;
+        __builtin_memcpy(&h, p, sizeof(h));
+        a = __low2float(h);
+        b = __high2float(h);
+    } else {
+        float2 f;
+        __builtin_memcpy(&f, p, sizeof(f));
+        a = f.x;
+        b = f.y;
+    }
+}
 
 template <typename T>
 __device__ __forceinline__ void issue3(const T* __restrict__ src, const SGeom& s, const float* m,
                                        float lx, float ly, float lz, bool inside, Pending3& p) {
-    p.ok = 0;
-    if (!inside) return;
     const float px = fmaf(m[2], lz, fmaf(m[1], ly, m[0] * lx)) + m[3];
     const float py = fmaf(m[6], lz, fmaf(m[5], ly, m[4] * lx)) + m[7];
     const float pz = fmaf(m[10], lz, fmaf(m[9], ly, m[8] * lx)) + m[11];
     float vx, vy, vz;
     to_voxel(s, px, py, pz, vx, vy, vz);
-    if (!(vx >= 0.0f && vx <= (float)(s.X - 1) && vy >= 0.0f && vy <= (float)(s.Y - 1) &&
-          vz >= 0.0f && vz <= (float)(s.Z - 1)))
-        return;
+    const float xm = (float)(s.X - 1), ym = (float)(s.Y - 1), zm = (float)(s.Z - 1);
+    p.ok = inside && vx >= 0.0f && vx <= xm && vy >= 0.0f && vy <= ym && vz >= 0.0f && vz <= zm;
+    vx = fminf(fmaxf(vx, 0.0f), xm);
+    vy = fminf(fmaxf(vy, 0.0f), ym);
+    vz = fminf(fmaxf(vz, 0.0f), zm);
     const float fx = floorf(vx), fy = floorf(vy), fz = floorf(vz);
-    const int x0 = (int)fx, y0 = (int)fy, z0 = (int)fz;
-    const int y1 = (int)ceilf(vy), z1 = (int)ceilf(vz);
-    const bool xs = (fx == vx);
+    const unsigned x0 = (unsigned)fx, y0 = (unsigned)fy, z0 = (unsigned)fz;
+    const unsigned y1 = (unsigned)ceilf(vy), z1 = (unsigned)ceilf(vz);
+    p.xs = (fx == vx);
     p.rx = vx - fx;
     p.ry = vy - fy;
     p.rz = vz - fz;
-    const int sxy = s.X * s.Y;
-    ld2(src, x0 + y0 * s.X + z0 * sxy, xs, p.a0, p.a1);
-    ld2(src, x0 + y0 * s.X + z1 * sxy, xs, p.b0, p.b1);
-    ld2(src, x0 + y1 * s.X + z0 * sxy, xs, p.c0, p.c1);
-    ld2(src, x0 + y1 * s.X + z1 * sxy, xs, p.d0, p.d1);
-    p.ok = 1;
+    const unsigned X = (unsigned)s.X, sxy = (unsigned)(s.X * s.Y);
+    ld2_off(src, x0 + y0 * X + z0 * sxy, p.a0, p.a1);
+    ld2_off(src, x0 + y0 * X + z1 * sxy, p.b0, p.b1);
+    ld2_off(src, x0 + y1 * X + z0 * sxy, p.c0, p.c1);
+    ld2_off(src, x0 + y1 * X + z1 * sxy, p.d0, p.d1);
 }
 
 __device__ __forceinline__ float finish3(const Pending3& p) {
-    if (!p.ok) return 0.0f;
+    // on an integer x the reference reads x0 twice (ceil == floor)
+    const float a1 = p.xs ? p.a0 : p.a1, b1 = p.xs ? p.b0 : p.b1;
+    const float c1 = p.xs ? p.c0 : p.c1, d1 = p.xs ? p.d0 : p.d1;
     const float lu = fmaf(p.rz, p.b0 - p.a0, p.a0);
-    const float ru = fmaf(p.rz, p.b1 - p.a1, p.a1);
+    const float ru = fmaf(p.rz, b1 - a1, a1);
     const float lb = fmaf(p.rz, p.d0 - p.c0, p.c0);
-    const float rb = fmaf(p.rz, p.d1 - p.c1, p.c1);
+    const float rb = fmaf(p.rz, d1 - c1, c1);
     const float yl = fmaf(p.ry, lb - lu, lu);
     const float yr = fmaf(p.ry, rb - ru, ru);
-    return fmaf(p.rx, yr - yl, yl);
+    return p.ok ? fmaf(p.rx, yr - yl, yl) : 0.0f;
 }
 
 // Interpolate_2Dkernel sample (interpolator.cu:186-230): bilinear, y then x. The 2x3
@@ -286,6 +308,20 @@ __device__ __forceinline__ bool combine_static_u(const double* S, double mu, dou
     const double e = explained_var(cii, cgg, cig, cyi, cyg, S[2], S[3]);
     weight_measure(cyy, e, inv_n1, w_out, m_out);
     return true;
+}
+
+// Work-group -> (tile, pose) on a 1D grid of roundup(tiles, 8) x npose work-groups.
+// Work-groups go to the 8 XCDs round-robin by linear id; this mapping keeps all poses of
+// one tile on the same XCD and adjacent in time (32 poses fill that XCD's 32 CUs), so the
+// tile's static data (ultrasound statistics, fixed image) and the overlapping source
+// footprint are fetched into that XCD's L2 once and shared by every pose. Returns false
+// for the padding work-groups (tile >= tiles), which exit at once.
+__device__ __forceinline__ bool map_block(int tiles, int npose, int& tile, int& pose) {
+    const int w = blockIdx.x;
+    const int xcd = w % kXcds, slot = w / kXcds;
+    pose = slot % npose;
+    tile = (slot / npose) * kXcds + xcd;
+    return tile < tiles;
 }
 
 template <int NTHR>
@@ -431,13 +467,15 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
     __shared__ int s_xcnt[CNT ? PY * XS : 1];
     __shared__ double s_red[2 * NTHR / 64];
 
+    int tile, pose;
+    if (!map_block(a.tiles, a.npose, tile, pose)) return;  // whole work-group
     const int tid = threadIdx.x;
-    const int tx = blockIdx.x % a.ntx, ty = blockIdx.x / a.ntx;
+    const int tcol = tile % (a.ntx * a.nty), zc = tile / (a.ntx * a.nty);
+    const int tx = tcol % a.ntx, ty = tcol / a.ntx;
     const int x0 = R + tx * TX, y0 = R + ty * TY;
-    const int z0 = R + blockIdx.y * a.zchunk;
+    const int z0 = R + zc * a.zchunk;
     const int zend = min(z0 + a.zchunk, a.v.Z - R);
     const int nout = zend - z0;
-    const int pose = blockIdx.z;
     const int X = a.v.X, Y = a.v.Y, Z = a.v.Z;
     const T* __restrict__ src = static_cast<const T*>(a.src);
     float m[12];
@@ -475,9 +513,14 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
             const int lx = i % PX, ly = i / PX;
             const int gx = x0 - R + lx, gy = y0 - R + ly;
             const bool in = i < NP && gx < X && gy < Y && zp >= 0 && zp < Z;
-            const int vidx = (zp * Y + gy) * X + gx;
-            pre0[j] = in ? a.stat_img[vidx] : 0.0f;
-            if (!MOVI) pre1[j] = in ? a.stat_grad[vidx] : 0.0f;
+            // unconditional load from a valid index, selected afterwards: no branch
+            const unsigned vidx = in ? (unsigned)((zp * Y + gy) * X + gx) : 0u;
+            const float v0 = a.stat_img[vidx];
+            pre0[j] = in ? v0 : 0.0f;
+            if (!MOVI) {
+                const float v1 = a.stat_grad[vidx];
+                pre1[j] = in ? v1 : 0.0f;
+            }
         }
     };
     // static ultrasound statistics of this thread's output voxel
@@ -486,10 +529,11 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
     double mu_pre = 0.0, cyy_pre = -1.0;
     auto issue_stats = [&](int zo) {
         if (MOVI) {
-            const int vo = (zo * Y + (y0 + oy)) * X + (x0 + ox);
             const bool ok = col_valid && zo >= z0 && zo < zend;
-            mu_pre = ok ? a.ustat_mean[vo] : 0.0;
-            cyy_pre = ok ? a.ustat_cyy[vo] : -1.0;
+            const unsigned vo = ok ? (unsigned)((zo * Y + (y0 + oy)) * X + (x0 + ox)) : 0u;
+            const double mu = a.ustat_mean[vo], cyy = a.ustat_cyy[vo];
+            mu_pre = ok ? mu : 0.0;
+            cyy_pre = ok ? cyy : -1.0;
         }
     };
 
@@ -505,7 +549,6 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
     }
     double ring[W][NC];
     int ringc[W];
-    double S[NC];
     int SC = 0;
 #pragma unroll
     for (int k = 0; k < W; ++k) {
@@ -513,9 +556,8 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
 #pragma unroll
         for (int c = 0; c < NC; ++c) ring[k][c] = 0.0;
     }
-#pragma unroll
-    for (int c = 0; c < NC; ++c) S[c] = 0.0;
     double acc_m = 0.0, acc_w = 0.0;
+    const int cols = a.ntx * a.nty;
 
     issue_plane(zw0);
     issue_static(z0 - R);      // products plane of the first products step
@@ -533,7 +575,8 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
 #pragma unroll
             for (int j = 0; j < SPT; ++j) {
                 const int i = tid + j * NTHR;
-                if (i < NI) s_in[i] = finish3(pend[j]);
+                const float v = finish3(pend[j]);
+                if (i < NI) s_in[i] = v;
             }
             if (s + 1 < nsteps) issue_plane(zw0 + s + 1);
             float cur0[PPT], cur1[PPT];
@@ -598,22 +641,28 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
             // 4. in-plane box sums
             box_xy<NC, CNT, TX, TY, NTHR>(s_prod, s_cnt, s_xp, s_xcnt, s_prod, s_cnt, PS, PY, W,
                                           tid);
-            // 5. z ring of plane sums, running patch sums, combine
-            constexpr int slot = (k - LAG + W) % W;
+            // 5. z ring of plane sums and combine. The patch sums add the W plane sums in
+            // z order (no running sum), so a voxel's value does not depend on where its
+            // work-group's z-chunk starts; integer counts can run.
+            constexpr int slot = (k - LAG + W) % W;  // ring slot of the plane stored now
             if (CNT) {
                 const int cc = s_cnt[tid];
                 SC += cc - ringc[slot];
                 ringc[slot] = cc;
             }
 #pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                const double v = s_prod[c * TX * TY + tid];
-                S[c] += v - ring[slot][c];
-                ring[slot][c] = v;
-            }
+            for (int c = 0; c < NC; ++c) ring[slot][c] = s_prod[c * TX * TY + tid];
             if (q >= 2 * R) {
                 const double mu = mu_pre, cyy = cyy_pre;
                 if (q + 1 < nout + 2 * R) issue_stats(zp - R + 1);
+                double S[NC];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    double t = ring[(slot + 1) % W][c];  // oldest plane
+#pragma unroll
+                    for (int j = 2; j <= W; ++j) t += ring[(slot + j) % W][c];
+                    S[c] = t;
+                }
                 if (col_valid) {
                     double w, meas;
                     bool ok;
@@ -629,12 +678,20 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
                         }
                     }
                 }
+                // Partial sums per block of kZBlock output planes and tile column, at
+                // fixed positions in z: every z-chunk (a multiple of kZBlock) and every
+                // batch size then adds the same numbers in the same order.
+                const int zo = zp - R;  // this step's output plane (block-uniform)
+                if ((zo - R + 1) % kZBlock == 0 || zo + 1 == zend) {
+                    double* dst = a.partials + ((size_t)pose * a.blocks_per_pose +
+                                                (size_t)((zo - R) / kZBlock) * cols + tcol) * 2;
+                    block_reduce_store<NTHR>(acc_m, acc_w, s_red, dst);
+                    acc_m = 0.0;
+                    acc_w = 0.0;
+                }
             }
         });
     }
-    double* dst = a.partials + ((size_t)pose * a.blocks_per_pose +
-                                (size_t)blockIdx.y * (a.ntx * a.nty) + blockIdx.x) * 2;
-    block_reduce_store<NTHR>(acc_m, acc_w, s_red, dst);
 }
 
 // ---------------------------------------------------------------------------
@@ -662,11 +719,12 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
     int* s_cnt = reinterpret_cast<int*>(s_in + uni);          // PX*PS
     int* s_xcnt = s_cnt + PX * PS;                            // PX*XS
 
+    int tile, pose;
+    if (!map_block(a.tiles, a.npose, tile, pose)) return;  // whole work-group
     const int tid = threadIdx.x;
-    const int tx = blockIdx.x % a.ntx, ty = blockIdx.x / a.ntx;
+    const int tcol = tile % (a.ntx * a.nty), frame = tile / (a.ntx * a.nty);
+    const int tx = tcol % a.ntx, ty = tcol / a.ntx;
     const int x0 = R + tx * kTile, y0 = R + ty * kTile;
-    const int frame = blockIdx.y;
-    const int pose = blockIdx.z;
     const int X = a.v.X, Y = a.v.Y;
     const int plane = frame * X * Y;
     float m[12];
@@ -758,8 +816,7 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
             }
         }
     }
-    double* dst = a.partials + ((size_t)pose * a.blocks_per_pose +
-                                (size_t)frame * (a.ntx * a.nty) + blockIdx.x) * 2;
+    double* dst = a.partials + ((size_t)pose * a.blocks_per_pose + tile) * 2;
     block_reduce_store<kThreads>(acc_m, acc_w, s_red, dst);
 }
 
@@ -990,6 +1047,12 @@ int fused_blocks_3d(const VGeom& v, int r, int zchunk, int* ntx, int* nty, int* 
     return *ntx * *nty * *nzc;
 }
 
+int fused_partials_3d(const VGeom& v, int r) {
+    int ntx, nty, nzc;
+    if (fused_blocks_3d(v, r, kZBlock, &ntx, &nty, &nzc) <= 0) return 0;
+    return ntx * nty * nzc;
+}
+
 int fused_blocks_2d(const VGeom& v, int r, int* ntx, int* nty) {
     const int ex = v.X - 2 * r, ey = v.Y - 2 * r;
     if (ex <= 0 || ey <= 0) { *ntx = *nty = 0; return 0; }
@@ -1000,15 +1063,18 @@ int fused_blocks_2d(const VGeom& v, int r, int* ntx, int* nty) {
 
 hipError_t launch_fused(const FusedArgs& a, int grid_ndim, int src_ndim, int movi, int n_poses,
                         hipStream_t st) {
-    if (a.blocks_per_pose == 0) return hipSuccess;
-    if (grid_ndim == 3) {
-        dim3 grid(a.ntx * a.nty, a.nzc, n_poses);
-        return a.src_f16 ? launch3r<__half>(a, movi, grid, st) : launch3r<float>(a, movi, grid, st);
-    }
-    if (a.r < 1 || a.r > kMaxR2) return hipErrorInvalidValue;
-    dim3 grid(a.ntx * a.nty, a.v.Z, n_poses);
-    return a.src_f16 ? launch2t<__half>(a, movi, src_ndim, grid, st)
-                     : launch2t<float>(a, movi, src_ndim, grid, st);
+    if (a.tiles == 0 || n_poses <= 0) return hipSuccess;
+    if (grid_ndim == 3 && (a.zchunk <= 0 || a.zchunk % kZBlock != 0)) return hipErrorInvalidValue;
+    FusedArgs b = a;
+    b.npose = n_poses;
+    const long long tiles8 = (a.tiles + kXcds - 1) / kXcds * (long long)kXcds;
+    if (tiles8 * n_poses > 0x7fffffffLL) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)(tiles8 * n_poses));
+    if (grid_ndim == 3)
+        return b.src_f16 ? launch3r<__half>(b, movi, grid, st) : launch3r<float>(b, movi, grid, st);
+    if (b.r < 1 || b.r > kMaxR2) return hipErrorInvalidValue;
+    return b.src_f16 ? launch2t<__half>(b, movi, src_ndim, grid, st)
+                     : launch2t<float>(b, movi, src_ndim, grid, st);
 }
 
 hipError_t launch_reduce(const double* partials, int blocks_per_pose, int n_poses, double* out,

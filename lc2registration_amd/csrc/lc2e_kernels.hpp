@@ -15,6 +15,8 @@ constexpr int kTile3Y = 16;    // one work-group of 512 threads, one column per 
 // Resident volumes carry this many zeroed bytes past their last voxel (paired x loads).
 constexpr size_t kVolumePad = 64;
 constexpr int kMaxR3 = 4;      // largest patch radius of the 3D fused kernel
+constexpr int kZBlock = 8;     // 3D partial sums per block of 8 output planes; z-chunks are multiples
+constexpr int kXcds = 8;       // MI355X: 8 XCDs, each with its own L2, 32 CUs each
 constexpr int kMaxR2 = 10;     // largest patch radius of the 2D fused kernel
 
 struct VGeom {  // virtual grid
@@ -51,19 +53,24 @@ struct FusedArgs {
     const double* ustat_mean;  // static ultrasound: patch mean per voxel
     const double* ustat_cyy;   // static ultrasound: centred patch energy, < 0 where w = 0
     const float* mats;       // [pose][mats_per_pose][12] VirtualToMoving, 3x4 row-major
-    double* partials;        // [pose][blocks_per_pose][2] (sum measure, sum w)
+    double* partials;        // [pose][blocks_per_pose][2] (sum measure, sum w); 3D: the
+                             // blocks are [z block of kZBlock planes][tile column]
     double* wmap;            // optional per-voxel w (single pose), may be null
     double* mmap;            // optional per-voxel measure, may be null
     int r;
     int zchunk;              // 3D: output planes per work-group
     int ntx, nty, nzc;       // tiles along x, y and z chunks (3D) / frames (2D)
-    int blocks_per_pose;
+    int blocks_per_pose;     // partial sums per pose
+    int tiles;               // work-groups per pose
     int mats_per_pose;
     int src_f16;             // moving image stored as half
+    int npose;               // poses in this launch (set by launch_fused)
 };
 
 // Number of work-groups one pose needs (0 if the grid has no valid patch centre).
 int fused_blocks_3d(const VGeom& v, int r, int zchunk, int* ntx, int* nty, int* nzc);
+// Partial sums per pose of the 3D kernel (tile columns x blocks of kZBlock planes).
+int fused_partials_3d(const VGeom& v, int r);
 int fused_blocks_2d(const VGeom& v, int r, int* ntx, int* nty);
 
 hipError_t launch_fused(const FusedArgs& a, int grid_ndim, int src_ndim, int high_res_is_moving,

@@ -74,6 +74,7 @@ def lib():
         L.lc2e_sync.argtypes = [vp]
         L.lc2e_debug_fetch.argtypes = [vp, C.c_int, fp, vp]
         L.lc2e_pair_set_tuning.argtypes = [vp, C.c_int]
+        L.lc2e_pair_set_tuning_n.argtypes = [vp, C.c_int, C.c_int]
         L.lc2e_pair_get_tuning.argtypes = [vp, C.POINTER(C.c_int)]
         L.lc2e_pair_use_graph.argtypes = [vp, C.c_int]
         L.lc2e_pair_autotune.argtypes = [vp, C.c_int, fp, C.c_int, C.POINTER(C.c_int),
@@ -223,7 +224,12 @@ class Pair:
         return out
 
     def set_tuning(self, zchunk: int):
+        """Force the 3D z-chunk for every launch (0: automatic)."""
         _check(lib().lc2e_pair_set_tuning(self.h, zchunk))
+
+    def set_tuning_n(self, n: int, zchunk: int):
+        """z-chunk for launches of exactly n poses (0: automatic)."""
+        _check(lib().lc2e_pair_set_tuning_n(self.h, n, zchunk))
 
     def get_tuning(self) -> int:
         z = C.c_int()

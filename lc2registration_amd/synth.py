@@ -244,3 +244,72 @@ def perturb(pose, rng, ndim, scale=1.0):
                       rng.uniform(-0.01, 0.01), rng.uniform(-0.01, 0.01),
                       rng.uniform(-0.01, 0.01)])
     return pose + scale * d
+
+
+def _frame_poses(n_frames, width_mm, depth_mm, centre, sweep_mm, rng):
+    """Tracked probe poses of a smooth seeded sweep: one 4x4 per frame mapping frame-plane
+    coordinates (u, v, 0) in mm (u across the probe face, v along depth) to world mm.
+    The probe face moves along world z over sweep_mm, with slow seeded tilts (<= 4 deg)
+    and a lateral wobble (<= 3 mm)."""
+    t = np.linspace(0.0, 1.0, n_frames)
+    ph = rng.uniform(0, 2 * np.pi, 4)
+    out = np.zeros((n_frames, 4, 4))
+    for k, s in enumerate(t):
+        ax = 4.0 * np.sin(2 * np.pi * s + ph[0])          # tilt about the sweep direction
+        az = 4.0 * np.sin(np.pi * s + ph[1])              # in-plane rotation
+        wob = 3.0 * np.sin(2 * np.pi * s + ph[2])
+        R = _rotation_xyz(ax, 0.0, az)
+        face = np.array([centre[0] - width_mm / 2 + wob,
+                         centre[1] - depth_mm / 2,
+                         centre[2] - sweep_mm / 2 + sweep_mm * s])
+        out[k, :3, :3] = R
+        out[k, :3, 3] = face
+        out[k, 3, 3] = 1.0
+    return out
+
+
+def _rotation_xyz(rx, ry, rz):
+    cx, sx = np.cos(np.radians(rx)), np.sin(np.radians(rx))
+    cy, sy = np.cos(np.radians(ry)), np.sin(np.radians(ry))
+    cz, sz = np.cos(np.radians(rz)), np.sin(np.radians(rz))
+    Rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    Ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
+    Rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+    return Rz @ Ry @ Rx
+
+
+def sweep_matrices(pose, frames, centre):
+    """Per-frame VirtualToMoving matrices (n_frames x 3 x 4, float32) for a 6-DoF rigid
+    registration pose (TX, TY, TZ mm, RX, RY, RZ deg) about `centre`: M_k = A(pose) F_k."""
+    A = geo.affine_3d(np.r_[np.asarray(pose, np.float64)[:6], 1, 1, 1, 0, 0, 0], centre,
+                      np.zeros(3)).astype(np.float64)
+    A4 = np.vstack([A, [0, 0, 0, 1]])
+    return np.stack([(A4 @ F)[:3] for F in frames]).astype(np.float32)
+
+
+def config2(seed=2, mr_dims=(320, 320, 256), n_frames=128, frame_dims=(512, 384),
+            frame_spacing=0.3, sweep_mm=100.0, patch_radius=3):
+    """Tracked 2D ultrasound sweep against an MR volume (2D-to-3D): MR phantom
+    (1 mm, moving, stored f16 by the caller) and n_frames uint8 frames (frame_spacing mm)
+    simulated from the MR on each frame plane at a seeded 6-DoF error (+/-5 mm, +/-5 deg).
+    The frames form the fixed stack and define the grid (one 2D LC2 per frame, summed)."""
+    mr, lab, rng = phantom_3d(mr_dims, 1.0, seed=seed)
+    W, H = frame_dims
+    width, depth = W * frame_spacing, H * frame_spacing
+    centre = np.array(mr_dims, np.float64) / 2
+    frames = _frame_poses(n_frames, width, depth, centre, sweep_mm, rng)
+    pose = np.concatenate([rng.uniform(-5, 5, 3), rng.uniform(-5, 5, 3)])
+    M = sweep_matrices(pose, frames, centre).astype(np.float64)
+    vv, uu = np.mgrid[0:H, 0:W].astype(np.float64)
+    L = np.stack([uu * frame_spacing, vv * frame_spacing, np.zeros_like(uu), np.ones_like(uu)])
+    stack = np.zeros((n_frames, H, W), np.uint8)
+    for k in range(n_frames):
+        P = np.tensordot(M[k], L, axes=1)               # 3 x H x W, MR mm = voxels
+        coords = np.stack([P[2], P[1], P[0]])
+        I = _sample(mr.astype(np.float64), coords)
+        labs = _sample(lab, coords, order=0).astype(np.int32)
+        stack[k] = quantize_u8(simulate_us(I, labs, rng, 2))
+    fixed = Image(stack, (W, H, n_frames), (frame_spacing, frame_spacing, 1.0), (0.0, 0.0, 0.0))
+    moving = Image(mr, tuple(mr_dims), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0))
+    return Workload("sweep_2d_to_3d", fixed, moving, (frame_spacing, frame_spacing), patch_radius,
+                    True, False, 2, pose, extra={"frames": frames, "centre": centre})

@@ -263,12 +263,25 @@ def test_graph_replay_matches_eager(engine, w3):
     p.close()
 
 
-@pytest.mark.parametrize("zchunk", [4, 9, 32])
-def test_zchunk_does_not_change_result_beyond_rounding(engine, w3, zchunk):
+def test_zchunk_and_batch_size_do_not_change_result(engine, w3):
+    """The z-chunk (rounded up to a multiple of 8 planes) and the batch size change how the
+    3D grid is split over work-groups, never the value: LC2 is bit-identical."""
     p = _pair_for(engine, w3)
-    M = p.pose_to_matrix(w3.true_pose)
-    ref = _oracle_eval(w3, M)[0].lc2
-    p.set_tuning(zchunk)
-    assert p.get_tuning() == zchunk
-    assert abs(p.eval(M) - ref) < LC2_TOL
+    rng = np.random.default_rng(11)
+    mats = np.stack([p.pose_to_matrix(w3.true_pose)] +
+                    [p.pose_to_matrix(synth.perturb(w3.true_pose, rng, 3)) for _ in range(4)])
+    ref = _oracle_eval(w3, mats[0])[0].lc2
+    base = p.eval_batch(mats).lc2
+    assert abs(base[0] - ref) < LC2_TOL
+    for zchunk, expect in ((4, 8), (9, 16), (32, 32), (1000, 1000 + (-1000) % 8)):
+        p.set_tuning(zchunk)
+        assert p.get_tuning() == expect
+        assert np.array_equal(p.eval_batch(mats).lc2, base), zchunk
+        assert p.eval(mats[2]) == base[2]
+    p.set_tuning(0)
+    p.set_tuning_n(5, 16)
+    assert np.array_equal(p.eval_batch(mats).lc2, base)
+    assert p.get_tuning() == 16
+    single = [p.eval(m) for m in mats]     # single poses: the model's chunk
+    assert np.array_equal(np.array(single), base)
     p.close()
