@@ -75,6 +75,83 @@ def cpu_baseline(w, M, budget_s):
             "exact_fp64_evals_per_s": out["exact_fp64"][0]}
 
 
+def sweep_figures(eng, E, synth, steps=3):
+    """Tracked 2D sweep against an MR volume (2D-to-3D): 128 uint8 frames of 512x384
+    (0.3 mm) against MR 320x320x256 stored f16, patch 7x7, 64 candidate poses per launch,
+    one matrix per frame and pose. Returns whole-sweep evaluations/s and pixel-poses/s."""
+    t0 = time.time()
+    w = synth.config2()
+    log(f"sweep workload synthesised in {time.time() - t0:.1f}s")
+    f, m = w.fixed, w.moving
+    fv = eng.upload(f.data, f.spacing, f.origin)
+    mv = eng.upload(m.data, m.spacing, m.origin, E.F16)
+    p = eng.pair(fv, mv, w.resampled_spacing, True, False, w.patch_radius, 2)
+    p.set_per_frame(True)
+    rng = np.random.default_rng(7)
+    poses = [w.true_pose] + [w.true_pose + np.r_[rng.uniform(-2, 2, 3), rng.uniform(-2, 2, 3)]
+                             for _ in range(63)]
+    mats = np.stack([synth.sweep_matrices(q, w.extra["frames"], w.extra["centre"]) for q in poses])
+    P = len(mats)
+    res = p.eval_batch(mats)
+    eng.sync()
+    p.profile(True)
+    t = time.perf_counter()
+    for _ in range(steps):
+        res = p.eval_batch(mats)
+    dt = (time.perf_counter() - t) / steps
+    kms, nl = p.profile_read()
+    p.profile(False)
+    X, Y, F = p.virt_dims
+    px = P * F * X * Y
+    out = {"sweep_evals_per_s": P / dt, "sweep_pixel_poses_per_s": px / dt,
+           "sweep_kernel_ms_per_launch": kms / max(nl, 1),
+           "sweep_config": f"{F} frames {X}x{Y} u8 (0.3 mm) vs MR 320x320x256 f16, 7x7 patch, "
+                           f"{P} poses per launch",
+           "sweep_lc2_at_true_pose": float(res.lc2[0])}
+    log(f"sweep: {out['sweep_evals_per_s']:.1f} whole-sweep evals/s "
+        f"({out['sweep_pixel_poses_per_s'] / 1e9:.2f} G pixel-poses/s)")
+    p.close()
+    fv.close()
+    mv.close()
+    return out
+
+
+def registration_latency_figures(synth, n_runs=8):
+    """Whole 2D registrations through the product path (lc2reg_register_yaml: YAML and raw
+    files in, DIRECT stage + two BOBYQA-stage rounds on the GPU, result YAML and log out)
+    on seeded 256x256 affine cases (7x7 patch). Returns p50 / p99 wall time and the
+    largest parameter error against the generating pose."""
+    import tempfile
+    from lc2registration_amd import registration as REG
+    times, errs = [], []
+    with tempfile.TemporaryDirectory() as tmp:
+        for seed in range(n_runs):
+            w = synth.config0(seed=seed)
+            f, m = w.fixed, w.moving
+            fp, mp = os.path.join(tmp, f"f{seed}.raw"), os.path.join(tmp, f"m{seed}.raw")
+            f.data.tofile(fp)
+            m.data.tofile(mp)
+            cfg = REG.write_parameters_yaml(
+                os.path.join(tmp, f"p{seed}.yaml"), 2,
+                (fp, "float32", f.dims, f.spacing, f.origin), (mp, "uint8", m.dims, m.spacing, m.origin),
+                "Fixed", w.resampled_spacing, w.patch_radius, tmp, False, [0, 0, 0, 1, 1, 0],
+                [-15, -15, -15, 0.9, 0.9, -0.1], [15, 15, 15, 1.1, 1.1, 0.1], direct_maxeval=50,
+                rounds=[((True, True, False, False), 1e-4), ((True, True, True, True), 1e-4)])
+            t = time.perf_counter()
+            x, lc2, nev, _ = REG.register_yaml("2D", cfg, write_images=False)
+            times.append(time.perf_counter() - t)
+            errs.append(float(np.abs(x - w.true_pose).max()))
+    ms = np.array(times[1:] if len(times) > 1 else times) * 1e3   # first run pays set-up
+    out = {"registration_2d_p50_ms": float(np.percentile(ms, 50)),
+           "registration_2d_p99_ms": float(np.percentile(ms, 99)),
+           "registration_2d_max_param_error": float(max(errs)),
+           "registration_2d_config": "256x256 f32 fixed vs u8 moving affine, 7x7 patch, DIRECT 50 "
+                                     "(translation) + BOBYQA-stage rounds (T+R, all), from identity"}
+    log(f"2D registration latency p50 {out['registration_2d_p50_ms']:.1f} ms, "
+        f"p99 {out['registration_2d_p99_ms']:.1f} ms, max parameter error {max(errs):.3f}")
+    return out
+
+
 def calibrate_copy(nbytes):
     """A copy kernel of known size: under rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per
     dispatch, divided by nbytes, give the factor that turns those counters into bytes."""
@@ -193,6 +270,8 @@ def main():
         mv16.close()
         log(f"single-pose fp32 {extra['single_pose_fp32_evals_per_s']:.1f} evals/s, "
             f"batched fp16 {extra['batched_fp16_evals_per_s']:.1f} evals/s")
+        extra.update(sweep_figures(eng, E, synth))
+        extra.update(registration_latency_figures(synth))
 
     cpu = None
     if not args.no_cpu_baseline:

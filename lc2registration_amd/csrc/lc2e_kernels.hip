@@ -17,8 +17,8 @@
 //
 // 3D structure (k_fused3d): a 512-thread work-group owns a 32x16 column of
 // output voxels over `zchunk` planes and marches along z. Per input plane:
-// warp the plane (+ Sobel halo) into LDS, separable Sobel row pass in LDS,
-// column pass in registers (3-plane ring per position), FP64 products into LDS,
+// warp the plane (+ Sobel halo) into LDS, separable Sobel of each 3x3 neighbourhood
+// with a 3-plane register ring along z, FP64 products into LDS,
 // sliding x then y box sums in LDS, and a (2R+1)-plane ring of plane sums in
 // registers gives the z box sum; complete patch sums are combined in FP64 and
 // accumulated per thread. No intermediate volume touches HBM.
@@ -383,28 +383,49 @@ __device__ __forceinline__ void store_products(double* s_prod, int* s_cnt, int n
     }
 }
 
-// Sliding box sums of width w along x then y, for NC FP64 channels and (CNT) one integer
-// count channel, over a region of py rows (row stride ps >= TX + w - 1) into TX x TY
-// outputs:
-//   s_prod[c][py][ps] -> s_xp[c][py][TX + 1] -> s_out[c][TY][TX]   (s_out may alias s_prod).
-// Rows (x pass) and columns (y pass) are cut into runs of 16 outputs, one task each;
-// neighbouring lanes take neighbouring rows (x pass) or columns (y pass).
-template <typename V>
-__device__ __forceinline__ void slide16(const V* in, int stride, int w, V* out, int ostride) {
-    V acc = 0;
-    for (int k = 0; k < w; ++k) acc += in[k * stride];
-    out[0] = acc;
+// Window sums of width W over a run of 16 outputs: out[o] = in[o] + ... + in[o + W - 1],
+// o = 0..15, inputs in[0 .. 14 + W] (elements `stride` apart). van Herk / Gil-Werman
+// form: the run is cut into blocks of W inputs and a window is the suffix of its block
+// plus the prefix of the next one, so a partial sum never holds a value from outside the
+// window. (A sliding sum that adds the entering and subtracts the leaving value keeps the
+// rounding residue of every large value that passed through it, which swamps windows of
+// small variance further along the run.) Each input is read once.
+template <int W, typename V>
+__device__ __forceinline__ void window16(const V* in, int stride, V* out, int ostride) {
+    constexpr int NB = (16 + W - 1) / W;  // blocks that hold outputs
+    constexpr int NIN = 15 + W;           // inputs of the run
+    V raw[W];
 #pragma unroll
-    for (int o = 1; o < 16; ++o) {
-        acc += in[(o + w - 1) * stride] - in[(o - 1) * stride];
-        out[o * ostride] = acc;
+    for (int j = 0; j < W; ++j) raw[j] = in[j * stride];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        V suf[W];  // suffix sums of block b
+        suf[W - 1] = raw[W - 1];
+#pragma unroll
+        for (int j = W - 2; j >= 0; --j) suf[j] = raw[j] + suf[j + 1];
+        V pre = 0;  // prefix of block b + 1: elements 0 .. j - 1 at output b * W + j
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            const int o = b * W + j;
+            if (o < 16) out[o * ostride] = j == 0 ? suf[0] : suf[j] + pre;
+            const int t = (b + 1) * W + j;
+            if (t < NIN) {
+                raw[j] = in[t * stride];
+                pre = j == 0 ? raw[j] : pre + raw[j];
+            }
+        }
     }
 }
 
-template <int NC, bool CNT, int TX, int TY, int NTHR>
+// Box sums of width W along x then y, for NC FP64 channels and (CNT) one integer count
+// channel, over a region of py rows (row stride ps >= TX + W - 1) into TX x TY outputs:
+//   s_prod[c][py][ps] -> s_xp[c][py][TX + 1] -> s_out[c][TY][TX]   (s_out may alias s_prod).
+// Rows (x pass) and columns (y pass) are cut into runs of 16 outputs, one task each;
+// neighbouring lanes take neighbouring rows (x pass) or columns (y pass).
+template <int NC, bool CNT, int TX, int TY, int NTHR, int W>
 __device__ __forceinline__ void box_xy(const double* s_prod, const int* s_cnt, double* s_xp,
                                        int* s_xcnt, double* s_out, int* s_ocnt, int ps, int py,
-                                       int w, int tid) {
+                                       int tid) {
     static_assert(TX % 16 == 0 && TY % 16 == 0, "tile edges are multiples of 16");
     constexpr int NT = NC + (CNT ? 1 : 0);
     constexpr int SX = TX / 16, SY = TY / 16;
@@ -414,22 +435,35 @@ __device__ __forceinline__ void box_xy(const double* s_prod, const int* s_cnt, d
         const int row = t % py, rest = t / py;
         const int seg = rest % SX, c = rest / SX;
         if (c < NC)
-            slide16(s_prod + c * np + row * ps + seg * 16, 1, w,
-                    s_xp + (c * py + row) * XS + seg * 16, 1);
+            window16<W>(s_prod + c * np + row * ps + seg * 16, 1,
+                        s_xp + (c * py + row) * XS + seg * 16, 1);
         else
-            slide16(s_cnt + row * ps + seg * 16, 1, w, s_xcnt + row * XS + seg * 16, 1);
+            window16<W>(s_cnt + row * ps + seg * 16, 1, s_xcnt + row * XS + seg * 16, 1);
     }
     __syncthreads();
     for (int t = tid; t < NT * TX * SY; t += NTHR) {
         const int col = t % TX, rest = t / TX;
         const int seg = rest % SY, c = rest / SY;
         if (c < NC)
-            slide16(s_xp + (c * py + seg * 16) * XS + col, XS, w,
-                    s_out + c * TX * TY + seg * 16 * TX + col, TX);
+            window16<W>(s_xp + (c * py + seg * 16) * XS + col, XS,
+                        s_out + c * TX * TY + seg * 16 * TX + col, TX);
         else
-            slide16(s_xcnt + seg * 16 * XS + col, XS, w, s_ocnt + seg * 16 * TX + col, TX);
+            window16<W>(s_xcnt + seg * 16 * XS + col, XS, s_ocnt + seg * 16 * TX + col, TX);
     }
     __syncthreads();
+}
+
+// box_xy for a patch radius known only at run time (the 2D kernel), 1 <= r <= kMaxR2.
+template <int NC, bool CNT, int TX, int TY, int NTHR, int RR = 1>
+__device__ __forceinline__ void box_xy_r(int r, const double* s_prod, const int* s_cnt,
+                                         double* s_xp, int* s_xcnt, double* s_out, int* s_ocnt,
+                                         int ps, int py, int tid) {
+    if (r == RR)
+        box_xy<NC, CNT, TX, TY, NTHR, 2 * RR + 1>(s_prod, s_cnt, s_xp, s_xcnt, s_out, s_ocnt, ps,
+                                                  py, tid);
+    else if constexpr (RR < kMaxR2)
+        box_xy_r<NC, CNT, TX, TY, NTHR, RR + 1>(r, s_prod, s_cnt, s_xp, s_xcnt, s_out, s_ocnt, ps,
+                                                py, tid);
 }
 
 // ---------------------------------------------------------------------------
@@ -459,8 +493,6 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
     constexpr bool CNT = !MOVI;
 
     __shared__ float s_in[NI];
-    __shared__ float s_s[MOVI ? PX * IYE : 1];
-    __shared__ float s_d[MOVI ? PX * IYE : 1];
     __shared__ double s_prod[NC * PY * PS];
     __shared__ int s_cnt[CNT ? PY * PS : 1];
     __shared__ double s_xp[NC * PY * XS];
@@ -513,14 +545,9 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
             const int lx = i % PX, ly = i / PX;
             const int gx = x0 - R + lx, gy = y0 - R + ly;
             const bool in = i < NP && gx < X && gy < Y && zp >= 0 && zp < Z;
-            // unconditional load from a valid index, selected afterwards: no branch
-            const unsigned vidx = in ? (unsigned)((zp * Y + gy) * X + gx) : 0u;
-            const float v0 = a.stat_img[vidx];
-            pre0[j] = in ? v0 : 0.0f;
-            if (!MOVI) {
-                const float v1 = a.stat_grad[vidx];
-                pre1[j] = in ? v1 : 0.0f;
-            }
+            const int vidx = (zp * Y + gy) * X + gx;
+            pre0[j] = in ? a.stat_img[vidx] : 0.0f;
+            if (!MOVI) pre1[j] = in ? a.stat_grad[vidx] : 0.0f;
         }
     };
     // static ultrasound statistics of this thread's output voxel
@@ -529,11 +556,10 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
     double mu_pre = 0.0, cyy_pre = -1.0;
     auto issue_stats = [&](int zo) {
         if (MOVI) {
+            const int vo = (zo * Y + (y0 + oy)) * X + (x0 + ox);
             const bool ok = col_valid && zo >= z0 && zo < zend;
-            const unsigned vo = ok ? (unsigned)((zo * Y + (y0 + oy)) * X + (x0 + ox)) : 0u;
-            const double mu = a.ustat_mean[vo], cyy = a.ustat_cyy[vo];
-            mu_pre = ok ? mu : 0.0;
-            cyy_pre = ok ? cyy : -1.0;
+            mu_pre = ok ? a.ustat_mean[vo] : 0.0;
+            cyy_pre = ok ? a.ustat_cyy[vo] : -1.0;
         }
     };
 
@@ -587,36 +613,37 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
             }
             if (q >= 0 && q + 1 < nout + 2 * R) issue_static(zp + 1);
             __syncthreads();
-            // 2. Sobel row pass (smooth s and difference d along x)
-            if (MOVI) {
-                for (int i = tid; i < PX * IYE; i += NTHR) {
-                    const int lx = i % PX, ly = i / PX;
-                    const float* row = s_in + ly * IXE + lx;
-                    const float l = row[0], c = row[1], r = row[2];
-                    s_d[i] = r - l;
-                    s_s[i] = (l + 2.0f * c) + r;
-                }
-                __syncthreads();
-            }
-            // 3. column pass into the 3-plane ring, |grad| and products of plane zp
+            // 2. in-plane Sobel of the 3x3 neighbourhood (rows: smooth s = (l + 2c) + r and
+            // difference d = r - l along x; then A, Bx, By down the rows), a 3-plane ring
+            // in registers along z, |grad| and the products of plane zp
 #pragma unroll
             for (int j = 0; j < PPT; ++j) {
                 const int i = tid + j * NTHR;
                 if (i < NP) {
                     const int lx = i % PX, ly = i / PX;
                     if (MOVI) {
-                        const float s0v = s_s[ly * PX + lx], s1v = s_s[(ly + 1) * PX + lx],
-                                    s2v = s_s[(ly + 2) * PX + lx];
-                        const float d0v = s_d[ly * PX + lx], d1v = s_d[(ly + 1) * PX + lx],
-                                    d2v = s_d[(ly + 2) * PX + lx];
+                        // rows in order, accumulating A = (s0 + 2 s1) + s2,
+                        // Bx = (d0 + 2 d1) + d2 and By = s2 - s0 as they arrive
+                        const float* row = s_in + ly * IXE + lx;
+                        float l = row[0], c = row[1], r = row[2];
+                        const float s0 = (l + 2.0f * c) + r;
+                        float a = s0, bx = r - l;
+                        row += IXE;
+                        l = row[0], c = row[1], r = row[2];
+                        a = a + 2.0f * ((l + 2.0f * c) + r);
+                        bx = bx + 2.0f * (r - l);
+                        const float cc = c;
+                        row += IXE;
+                        l = row[0], c = row[1], r = row[2];
+                        const float s2 = (l + 2.0f * c) + r;
                         tA[j][0] = tA[j][1]; tA[j][1] = tA[j][2];
                         tBx[j][0] = tBx[j][1]; tBx[j][1] = tBx[j][2];
                         tBy[j][0] = tBy[j][1]; tBy[j][1] = tBy[j][2];
-                        tA[j][2] = (s0v + 2.0f * s1v) + s2v;
-                        tBx[j][2] = (d0v + 2.0f * d1v) + d2v;
-                        tBy[j][2] = s2v - s0v;
+                        tA[j][2] = a + s2;
+                        tBx[j][2] = bx + (r - l);
+                        tBy[j][2] = s2 - s0;
                         Iprev[j] = Icur[j];
-                        Icur[j] = s_in[(ly + 1) * IXE + lx + 1];
+                        Icur[j] = cc;
                     }
                     if (q >= 0) {
                         float I, G, U;
@@ -636,12 +663,12 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
                     }
                 }
             }
-            __syncthreads();  // s_in / s_s / s_d reads done; products visible
+            __syncthreads();  // s_in reads done; products visible
             if (q < 0) return;  // block-uniform: Sobel ring warm-up only
-            // 4. in-plane box sums
-            box_xy<NC, CNT, TX, TY, NTHR>(s_prod, s_cnt, s_xp, s_xcnt, s_prod, s_cnt, PS, PY, W,
-                                          tid);
-            // 5. z ring of plane sums and combine. The patch sums add the W plane sums in
+            // 3. in-plane box sums
+            box_xy<NC, CNT, TX, TY, NTHR, W>(s_prod, s_cnt, s_xp, s_xcnt, s_prod, s_cnt, PS, PY,
+                                             tid);
+            // 4. z ring of plane sums and combine. The patch sums add the W plane sums in
             // z order (no running sum), so a voxel's value does not depend on where its
             // work-group's z-chunk starts; integer counts can run.
             constexpr int slot = (k - LAG + W) % W;  // ring slot of the plane stored now
@@ -707,15 +734,13 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
     const int PS = odd_stride(PX);                            // products row stride
     constexpr int XS = odd_stride(kTile + 1);                 // x-pass row stride
     const int NI = MOVI ? IXE * IXE : NP;
-    // The x-pass output s_xp shares its space with the warp / Sobel scratch (s_in, s_s,
-    // s_d): those are dead once the products are stored, before the box sums start.
-    const int uni = max(2 * NC * PX * XS, NI + 2 * PX * IXE);  // shared region, in floats
+    // The x-pass output s_xp shares its space with the warped tile s_in, which is dead
+    // once the products are stored, before the box sums start.
+    const int uni = max(2 * NC * PX * XS, NI);                // shared region, in floats
     double* s_prod = s_dyn;                                   // NC*PX*PS
     double* s_red = s_prod + NC * PX * PS;                    // 8
     double* s_xp = s_red + 8;                                 // NC*PX*XS
     float* s_in = reinterpret_cast<float*>(s_xp);             // NI
-    float* s_s = s_in + NI;                                   // PX*IXE
-    float* s_d = s_s + PX * IXE;                              // PX*IXE
     int* s_cnt = reinterpret_cast<int*>(s_in + uni);          // PX*PS
     int* s_xcnt = s_cnt + PX * PS;                            // PX*XS
 
@@ -761,16 +786,6 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
         }
     }
     __syncthreads();
-    if (MOVI) {
-        for (int i = tid; i < PX * IXE; i += kThreads) {
-            const int lx = i % PX, ly = i / PX;
-            const float* row = s_in + ly * IXE + lx;
-            const float l = row[0], c = row[1], r = row[2];
-            s_d[i] = r - l;
-            s_s[i] = (l + 2.0f * c) + r;
-        }
-        __syncthreads();
-    }
     for (int i = tid; i < NP; i += kThreads) {
         const int lx = i % PX, ly = i / PX;
         const int gx = x0 - R + lx, gy = y0 - R + ly;
@@ -778,13 +793,20 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
         const int vidx = plane + gy * X + gx;
         float I, G, U;
         if (MOVI) {
-            const float s0v = s_s[ly * PX + lx], s2v = s_s[(ly + 2) * PX + lx];
-            const float d0v = s_d[ly * PX + lx], d1v = s_d[(ly + 1) * PX + lx],
-                        d2v = s_d[(ly + 2) * PX + lx];
-            const float gxv = (d0v + 2.0f * d1v) + d2v;
-            const float gyv = s2v - s0v;
+            // 3x3 Sobel read from the warped tile: rows smooth / difference along x
+            float sv[3], dv[3], cc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const float* row = s_in + (ly + k) * IXE + lx;
+                const float l = row[0], c = row[1], r = row[2];
+                dv[k] = r - l;
+                sv[k] = (l + 2.0f * c) + r;
+                if (k == 1) cc = c;
+            }
+            const float gxv = (dv[0] + 2.0f * dv[1]) + dv[2];
+            const float gyv = sv[2] - sv[0];
             G = sqrtf(fmaf(gyv, gyv, gxv * gxv));
-            I = s_in[(ly + 1) * IXE + lx + 1];
+            I = cc;
             U = in ? a.stat_img[vidx] : 0.0f;
         } else {
             U = s_in[i];
@@ -794,8 +816,8 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
         store_products<NC>(s_prod, s_cnt, PX * PS, ly * PS + lx, I, G, U);
     }
     __syncthreads();
-    box_xy<NC, CNT, kTile, kTile, kThreads>(s_prod, s_cnt, s_xp, s_xcnt, s_prod, s_cnt, PS, PX, W,
-                                            tid);
+    box_xy_r<NC, CNT, kTile, kTile, kThreads>(R, s_prod, s_cnt, s_xp, s_xcnt, s_prod, s_cnt, PS, PX,
+                                              tid);
     const int ox = tid % kTile, oy = tid / kTile;
     double acc_m = 0.0, acc_w = 0.0;
     if ((x0 + ox < X - R) && (y0 + oy < Y - R)) {
@@ -991,7 +1013,7 @@ constexpr size_t lds_bytes_2d(int r) {
     // the layout of k_fused2d for the 9-channel (moving ultrasound) variant, the larger one
     const size_t PX = kTile + 2 * r, IXE = PX + 2;
     const size_t PS = odd_stride((int)PX), XS = odd_stride(kTile + 1);
-    const size_t uni = std::max(2 * NCH * PX * XS, IXE * IXE + 2 * PX * IXE);
+    const size_t uni = std::max(2 * NCH * PX * XS, IXE * IXE);
     return sizeof(double) * (NCH * PX * PS + 8) + sizeof(float) * uni +
            sizeof(int) * (PX * PS + PX * XS);
 }

@@ -285,3 +285,47 @@ def test_zchunk_and_batch_size_do_not_change_result(engine, w3):
     single = [p.eval(m) for m in mats]     # single poses: the model's chunk
     assert np.array_equal(np.array(single), base)
     p.close()
+
+
+def test_tracked_sweep_2d_to_3d_matches_oracle(engine):
+    """A small tracked sweep (per-frame matrices, MR stored f16) stage by stage against the
+    oracle's slice warp (f16-rounded MR), 2D Sobel and exact LC2, frame by frame."""
+    w = synth.config2(seed=3, mr_dims=(96, 96, 80), n_frames=5, frame_dims=(96, 64),
+                      frame_spacing=0.5, sweep_mm=20.0, patch_radius=3)
+    f, m = w.fixed, w.moving
+    fv = engine.upload(f.data, f.spacing, f.origin)
+    mv = engine.upload(m.data, m.spacing, m.origin, E.F16)
+    p = engine.pair(fv, mv, w.resampled_spacing, True, False, 3, 2)
+    p.set_per_frame(True)
+    mats = synth.sweep_matrices(w.true_pose, w.extra["frames"], w.extra["centre"])
+    wi = p.debug_fetch(E.STAGE_WARP_I, mats)
+    gr = p.debug_fetch(E.STAGE_GRAD, mats)
+    wu = p.debug_fetch(E.STAGE_WARP_US, mats)
+    wm_gpu = p.debug_fetch(E.STAGE_W, mats)
+    mm_gpu = p.debug_fetch(E.STAGE_MEASURE, mats)
+    r = p.eval_batch(mats[None])
+    mr16 = m.data.astype(np.float16).astype(np.float32)
+    mg = O.geom(m.dims, m.spacing, m.origin)
+    X, Y = f.dims[0], f.dims[1]
+    vg = O.geom((X, Y, 1), (f.spacing[0], f.spacing[1], 1.0), (0, 0, 0))
+    sm = sw = 0.0
+    for k in range(len(mats)):
+        I = O.warp_3d(mr16, mg, vg, mats[k].ravel())[0]
+        U = f.data[k].astype(np.float32)
+        assert np.array_equal(wi[k], I), k
+        assert np.array_equal(wu[k], U), k
+        G = O.sobel(I, "sep")
+        assert np.array_equal(gr[k], G), k
+        res, wm, mm = O.lc2_exact(U, I, G, 3, maps=True)
+        assert np.array_equal(wm_gpu[k] > 0, wm > 0), k
+        assert np.allclose(wm_gpu[k], wm, rtol=1e-9, atol=1e-12), k
+        assert np.allclose(mm_gpu[k], mm, rtol=1e-6, atol=1e-7), (k, np.abs(mm_gpu[k] - mm).max())
+        sm += res.sum_measure
+        sw += res.sum_w
+    assert sw > 0
+    assert abs(r.sum_w[0] - sw) < 1e-9 * sw
+    assert abs(r.sum_measure[0] - sm) < 1e-8 * sm
+    assert abs(r.lc2[0] - sm / sw) < 1e-8
+    p.close()
+    fv.close()
+    mv.close()

@@ -1,7 +1,6 @@
 """Registration layer on the GPU: YAML-driven runs through the C ABI (lc2reg.h) on the
 reference's kidney phantom and on a synthetic volume pair with a known pose, and the
 same optimizer driven by the engine and by the oracle."""
-import ctypes as C
 import json
 import os
 
@@ -9,8 +8,8 @@ import numpy as np
 import pytest
 import yaml
 
-from lc2registration_amd import engine as E
 from lc2registration_amd import geometry as geo
+from lc2registration_amd import registration as REG
 from lc2registration_amd import synth
 from oracle import oracle as O
 
@@ -18,83 +17,27 @@ pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 KIDNEY = os.path.join(HERE, "golden", "kidney")
-DP = C.POINTER(C.c_double)
-BATCH_FN = C.CFUNCTYPE(None, C.c_int, C.c_int, DP, DP, C.c_void_p)
-KEYS_3D = ["TX", "TY", "TZ", "RX", "RY", "RZ", "ScaleX", "ScaleY", "ScaleZ", "ShearX", "ShearY",
-           "ShearZ"]
-KEYS_2D = ["TX", "TY", "R", "ScaleX", "ScaleY", "Shear"]
+KEYS_3D = REG.KEYS_3D
 
 
 @pytest.fixture(scope="module")
 def reg():
-    L = C.CDLL(E.library_path())
-    L.lc2reg_last_error.restype = C.c_char_p
-    L.lc2reg_register_yaml.argtypes = [C.c_char_p, C.c_char_p, C.c_int, DP, DP,
-                                       C.POINTER(C.c_int), C.POINTER(C.c_float)]
-    L.lc2reg_evaluate_yaml.argtypes = [C.c_char_p, C.c_char_p, DP, C.c_int, DP]
-    L.lc2reg_minimize_qmtr.argtypes = [C.c_int, DP, DP, DP, C.c_double, C.c_int, BATCH_FN,
-                                       C.c_void_p, DP, DP, C.POINTER(C.c_int)]
-    return L
-
-
-def _p(a):
-    return a.ctypes.data_as(DP)
-
-
-def _flow(keys, vals):
-    return "{" + ", ".join(f"{k}: {v}" for k, v in zip(keys, vals)) + "}"
-
-
-def _bools(e):
-    return _flow(["Translation", "Rotation", "Scale", "Shear"],
-                 ["true" if b else "false" for b in e])
+    return REG.lib()
 
 
 def write_config(path, ndim, fixed, moving, reg_params, init, lower, upper, direct, rounds):
-    """A parameters*.yaml in the reference's current layout (config/parameters3d_affine.yaml,
-    config/parameters2d_affine.yaml). fixed / moving: (file, dtype, dims, spacing, origin)."""
-    ax = "xyz"[:ndim]
-    kind = "Volume" if ndim == 3 else "Image"
-    lines = ["InputParameters:"]
-    for tag, (fp, dt, dims, sp, org) in (("Fixed", fixed), ("Moving", moving)):
-        lines += [f"  {tag}{kind}:", f"    Dimension: {_flow(ax, dims)}",
-                  f"    Spacing: {_flow(ax, sp)}", f"    Origin: {_flow(ax, org)}",
-                  f"    FilePath: {fp}", "    Format: raw", f"    DataType: {dt}"]
     hr, rs, patch, out_dir, centred = reg_params
-    keys = KEYS_3D if ndim == 3 else KEYS_2D
-    lines += ["  RegistrationParameters:", f"    HighResolutionModality: {hr}", "    Type: Affine",
-              "    DisplayPattern: 0", f"    ResampledSpacing: {_flow(ax, rs)}",
-              f"    PatchSize: {patch}", f"    FinalRoundExportRootPath: {out_dir}",
-              f"    CenterOverlaid: {'true' if centred else 'false'}",
-              "  OptimizerSettings:", f"    InitialValues: {_flow(keys, init)}",
-              f"    LowerBounds: {_flow(keys, lower)}", f"    UpperBounds: {_flow(keys, upper)}",
-              "    DIRECT:", f"      Enabled: {'true' if direct else 'false'}",
-              f"      OptimizedEntries: {_bools((True, False, False, False))}",
-              f"      MaxEvaluations: {direct or 30}", "    BOBYQA:"]
-    for i, (entries, tol) in enumerate(rounds):
-        lines += [f"      {i}:", "        Enabled: true",
-                  f"        OptimizedEntries: {_bools(entries)}", f"        AbsTol: {tol}"]
-    with open(path, "w") as fh:
-        fh.write("\n".join(lines) + "\n")
+    return REG.write_parameters_yaml(path, ndim, fixed, moving, hr, rs, patch, out_dir, centred,
+                                     init, lower, upper, direct_maxeval=direct, rounds=rounds)
 
 
 def register(reg, mode, path, nparams, images=1):
-    x = np.zeros(nparams)
-    lc2, nev = C.c_double(), C.c_int()
-    mat = (C.c_float * 16)()
-    rc = reg.lc2reg_register_yaml(mode.encode(), path.encode(), images, _p(x), C.byref(lc2),
-                                  C.byref(nev), mat)
-    assert rc == 0, reg.lc2reg_last_error()
-    return x, lc2.value, nev.value, np.array(mat[:16])
+    x, lc2, nev, mat = REG.register_yaml(mode, path, bool(images))
+    return x, lc2, nev, mat.ravel()
 
 
 def evaluate(reg, mode, path, params):
-    params = np.ascontiguousarray(params, np.float64)
-    out = C.c_double()
-    rc = reg.lc2reg_evaluate_yaml(mode.encode(), path.encode(), _p(params), params.size,
-                                  C.byref(out))
-    assert rc == 0, reg.lc2reg_last_error()
-    return out.value
+    return REG.evaluate_yaml(mode, path, params)
 
 
 def consistent_pair(seed=11):
@@ -167,14 +110,7 @@ def test_yaml_registration_3d_recovers_known_pose(reg, tmp_path):
     assert os.path.exists(tmp_path / "fixed.nrrd") and os.path.exists(tmp_path / "moving_Registered.nrrd")
 
 
-def _objective(fn):
-    def cb(n, d, X, F, _):
-        xs = np.ctypeslib.as_array(X, shape=(n, d)).copy()
-        np.ctypeslib.as_array(F, shape=(n,))[:] = fn(xs)
-    return BATCH_FN(cb)
-
-
-def test_same_optimizer_engine_vs_oracle(engine, reg):
+def test_same_optimizer_engine_vs_oracle(engine):
     """The batched trust-region optimizer driven by the GPU LC2 and by the oracle's LC2
     on the same inputs ends at the same pose (0.02 mm / 0.02 deg) and value (1e-7), next
     to the generating pose. The two LC2s differ by ~1e-13, so the trajectories may part
@@ -199,14 +135,8 @@ def test_same_optimizer_engine_vs_oracle(engine, reg):
 
     x0 = base[:6] + np.array([1.0, -0.8, 0.6, 0.8, -0.6, 0.7])
     lb, ub = x0 - 4, x0 + 4
-    res = {}
-    for name, fn in (("gpu", gpu), ("host", host)):
-        xo, fo, n = np.zeros(6), C.c_double(), C.c_int()
-        cb = _objective(fn)
-        rc = reg.lc2reg_minimize_qmtr(6, _p(x0), _p(lb), _p(ub), 1e-4, 2000, cb, None, _p(xo),
-                                      C.byref(fo), C.byref(n))
-        assert rc == 0, reg.lc2reg_last_error()
-        res[name] = (xo, fo.value, n.value)
+    res = {name: REG.minimize_qmtr(fn, x0, lb, ub, 1e-4, 2000)
+           for name, fn in (("gpu", gpu), ("host", host))}
     print("engine", res["gpu"], "oracle", res["host"])
     assert np.abs(res["gpu"][0] - res["host"][0]).max() < 0.02
     assert abs(res["gpu"][1] - res["host"][1]) < 1e-7

@@ -145,3 +145,33 @@ def test_qmtr_rosenbrock(reg):
           C.byref(n))
     assert f.value < 1e-4
     assert np.allclose(x, [1.0, 1.0], atol=2e-2)
+
+
+def test_python_binding_and_config_writer(tmp_path):
+    """lc2registration_amd.registration: the batch minimizers from Python and the
+    reference-layout configuration writer (read back by PyYAML and by the C++ reader)."""
+    from lc2registration_amd import registration as REG
+    x, f, n = REG.minimize_qmtr(lambda X: ((X - [0.5, -1.0]) ** 2).sum(1), [0.0, 0.0],
+                                [-2, -2], [2, 2], 1e-8, 2000)
+    assert np.allclose(x, [0.5, -1.0], atol=1e-4) and f < 1e-8
+    x, f, n = REG.minimize_direct(lambda X: np.array([branin(p) for p in X]), [-5, 0], [10, 15], 200)
+    assert f < 0.397887 + 5e-3 and n <= 200
+    cfg = str(tmp_path / "p3.yaml")
+    REG.write_parameters_yaml(cfg, 3, ("/data/us.raw", "uint8", (192, 192, 160), (0.5,) * 3, (1, 2, 3)),
+                              ("/data/mr.raw", "float32", (256, 256, 192), (1.0,) * 3, (0, 0, 0)),
+                              "Moving", (0.5,) * 3, 3, str(tmp_path), False,
+                              [0] * 6 + [1] * 3 + [0] * 3, [-5] * 6 + [0.9] * 3 + [-0.1] * 3,
+                              [5] * 6 + [1.1] * 3 + [0.1] * 3, direct_maxeval=40,
+                              rounds=[((True, True, False, False), 1e-4),
+                                      ((True, True, True, True), 1e-5)])
+    y = yaml.safe_load(open(cfg))
+    ip = y["InputParameters"]
+    assert ip["FixedVolume"]["Dimension"] == {"x": 192, "y": 192, "z": 160}
+    assert ip["RegistrationParameters"]["HighResolutionModality"] == "Moving"
+    assert ip["OptimizerSettings"]["DIRECT"] == {
+        "Enabled": True, "MaxEvaluations": 40,
+        "OptimizedEntries": {"Translation": True, "Rotation": False, "Scale": False, "Shear": False}}
+    assert ip["OptimizerSettings"]["BOBYQA"][1]["AbsTol"] == 1e-5
+    out = str(tmp_path / "rt.yaml")
+    assert REG.lib().lc2reg_yaml_roundtrip(cfg.encode(), out.encode()) == 0
+    assert yaml.safe_load(open(out)) == y
