@@ -16,7 +16,9 @@
 //    GetSimilarityMeasure() runs warp + Sobel + LC2 in one launch.
 //  * Buffer getters return host copies (the reference's 3D getters return device
 //    vectors by value, cu_lc2_3d.cu:1335-1341, which makes its 3D LC2 read zeros).
-//  * patchSize 0 (mutual information, cu_lc2_3d.cu:1196-1301) is not provided: it throws.
+//  * patchSize 0 selects the normalized mutual information (cu_lc2_3d.cu:1196-1301,
+//    256 x 256 bins) for 3D grids, as the reference does; the optimizers then maximise it.
+//    2D grids reject patchSize 0 (the reference runs LC2 with an empty patch there).
 //  * The optimizers' NLopt calls are served by lc2opt::direct_l / lc2opt::qmtr
 //    (optim.hpp) and every stage evaluates its candidate points in batches.
 //  * Images come from raw files or NRRD (the reference reads through ITK).
@@ -140,9 +142,12 @@ public:
     // The engine pair behind the facade (valid after InitiateRegistrationHandle).
     lc2e_pair* pair() const { return pair_; }
     int patch_size() const { return patch_size_; }
+    // Patch radius of later evaluations (0: mutual information, 3D grids only).
+    void set_patch_size(int patchSize);
     // Current VirtualToMoving as 3x4 row-major.
     const float* current_matrix() const { return current_; }
-    // One batch of full parameter vectors (12 for 3D sources, 6 for 2D) -> LC2 values.
+    // One batch of full parameter vectors (12 for 3D sources, 6 for 2D) -> LC2 values (NMI
+    // when the patch radius is 0).
     int EvaluateBatch(const std::vector<std::vector<double>>& params, std::vector<double>& lc2);
     // Warped images on the virtual grid at the current pose (host copies).
     std::vector<float> FetchStage(lc2e_stage stage);

@@ -86,7 +86,8 @@ int lc2e_memory(lc2e_engine* e, size_t* used, size_t* budget);
 
 /* Upload a resident fixed or moving image. Replaces Interpolator*'s thrust copies of the
  * volumes (interpolator.cu:276, :315; :1616-1627 for raw files). `storage` is LC2E_F16,
- * LC2E_F32 or LC2E_STORE_AUTO (U8 -> F16, F16 -> F16, I16/F32 -> F32: all lossless). */
+ * LC2E_F32 or LC2E_STORE_AUTO (U8 -> F16, F16 -> F16, I16/F32 -> F32: all lossless).
+ * F32 data stored as F16 must lie within +-65504 (LC2E_ERR_INVALID otherwise). */
 int lc2e_volume_upload(lc2e_engine* e, const void* host, lc2e_dtype dtype, const lc2e_geom* geom,
                        lc2e_dtype storage, lc2e_volume** out);
 void lc2e_volume_destroy(lc2e_volume* v);
@@ -131,6 +132,17 @@ int lc2e_eval_batch(lc2e_pair* p, int n, const float* mats, double* lc2, double*
  * (NULL = the engine's stream). Nothing is synchronised. */
 int lc2e_eval_batch_async(lc2e_pair* p, int n, const float* d_mats, double* d_out, void* stream);
 int lc2e_sync(lc2e_engine* e);
+
+/* Normalized mutual information, the reference's patchSize == 0 metric
+ * (LC2_3D_class::GetMutualInformationSimilarityMetric, cu_lc2_3d.cu:1196-1301, reached from
+ * Interpolator3D::GetSimilarityMeasure, interpolator.cu:617-622; its callers maximise it,
+ * optimizers.cu:722-724, :855-858): for each of n poses, (H_US + H_HR) / H_joint of the
+ * two images on the virtual grid, with bins_us / bins_hr bins (2..256; the reference uses
+ * 256 and 256) spread over each image's minimum..maximum and bin index
+ * roundf((v - min) * (bins - 1) / (max - min)). Entropies are summed in FP64 (the
+ * reference: FP32). nmi[n]; 0 when an image is constant. */
+int lc2e_eval_mi_batch(lc2e_pair* p, int n, const float* mats, int bins_us, int bins_hr,
+                       double* nmi);
 
 /* Per-stage buffers for parity tests at one pose (the reference's buffer getters,
  * cu_lc2_3d.cuh:45-49, cu_lc2_2d.cuh:40-46). host receives virt voxels of float32

@@ -159,11 +159,21 @@ void InterpolatorBase::InitiateMappers() {}
 void InterpolatorBase::InitiateRegistrationHandle() {
     if (pair_) lc2e_pair_destroy(pair_);
     pair_ = nullptr;
+    if (patch_size_ == 0 && grid_ndim_ != 3)
+        throw std::runtime_error("patchSize 0 (mutual information) needs a 3D grid");
+    // a mutual-information pair still carries LC2 statistics; radius 1 keeps them cheap
     check(lc2e_pair_create(shared_engine(), fixed_.vol, moving_.vol, resampled_,
-                           HighResolutionModality == "Moving", Centered ? 1 : 0, patch_size_,
-                           grid_ndim_, &pair_),
+                           HighResolutionModality == "Moving", Centered ? 1 : 0,
+                           patch_size_ > 0 ? patch_size_ : 1, grid_ndim_, &pair_),
           "pair create");
     check(lc2e_pair_get_info(pair_, &info_), "pair info");
+}
+
+void InterpolatorBase::set_patch_size(int patchSize) {
+    if (patchSize < 0) throw std::runtime_error("negative patch size");
+    if (pair_ && patchSize == patch_size_) return;
+    patch_size_ = patchSize;
+    InitiateRegistrationHandle();
 }
 
 void InterpolatorBase::set_pose(const double* params, int n) {
@@ -172,15 +182,13 @@ void InterpolatorBase::set_pose(const double* params, int n) {
     ++counter;
 }
 
+// GetSimilarityMeasure (interpolator.cu:617-631): LC2 of the current pose, or the
+// normalized mutual information for patchSize 0 (interpolator.cu:620-622).
 double InterpolatorBase::similarity(int patchSize) {
-    if (patchSize == 0)
-        throw std::runtime_error("patchSize 0 selects mutual information, which this engine does not provide");
-    if (!pair_ || patchSize != patch_size_) {
-        patch_size_ = patchSize;
-        InitiateRegistrationHandle();
-    }
+    set_patch_size(patchSize);
     double l = 0.0;
-    check(lc2e_eval(pair_, current_, &l), "eval");
+    if (patchSize == 0) check(lc2e_eval_mi_batch(pair_, 1, current_, 256, 256, &l), "mutual information");
+    else check(lc2e_eval(pair_, current_, &l), "eval");
     return l;
 }
 
@@ -192,7 +200,9 @@ int InterpolatorBase::EvaluateBatch(const std::vector<std::vector<double>>& para
         check(lc2e_pose_to_matrix(pair_, params[i].data(), (int)params[i].size(), &mats[i * 12]),
               "pose");
     lc2.assign(params.size(), 0.0);
-    const int rc = lc2e_eval_batch(pair_, (int)params.size(), mats.data(), lc2.data(), nullptr, nullptr);
+    const int rc = patch_size_ == 0
+                       ? lc2e_eval_mi_batch(pair_, (int)params.size(), mats.data(), 256, 256, lc2.data())
+                       : lc2e_eval_batch(pair_, (int)params.size(), mats.data(), lc2.data(), nullptr, nullptr);
     check(rc, "eval batch");
     return rc;
 }
@@ -439,8 +449,15 @@ double LC2_3D_class::GetSimilarityMetric(const int, const int, const int, const 
     return l;
 }
 
-double LC2_3D_class::GetMutualInformationSimilarityMetric(const int, const int, const int, const int, const int) {
-    throw std::runtime_error("mutual information similarity is not provided by this engine");
+double LC2_3D_class::GetMutualInformationSimilarityMetric(const int, const int, const int,
+                                                         const int bins_usVolume,
+                                                         const int bins_highResoVolume) {
+    ensure(patch_ > 0 ? patch_ : 3);
+    const float ident[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    double v = 0.0;
+    if (lc2e_eval_mi_batch(pair_, 1, ident, bins_usVolume, bins_highResoVolume, &v) < 0)
+        throw std::runtime_error(lc2e_last_error());
+    return v;
 }
 
 std::vector<float> LC2_3D_class::GetUltrasoundBuffer() { return us_; }
@@ -610,7 +627,9 @@ void OptimizerBase::evaluate_batch(int n, int d, const double* X, double* F) {
         for (int i = 0; i < m; ++i) {
             ++EvalCounter;
             SimilarityMeasure = lc2[i];
-            F[base + i] = lc2[i] > 0.0 ? -std::log(lc2[i]) : std::numeric_limits<double>::infinity();
+            // optimizers.cu:722-727: mutual information is maximised as is, LC2 as -log
+            if (interp_->patch_size() == 0) F[base + i] = -lc2[i];
+            else F[base + i] = lc2[i] > 0.0 ? -std::log(lc2[i]) : std::numeric_limits<double>::infinity();
             log_eval(params[i], lc2[i]);
         }
     }
@@ -639,11 +658,13 @@ void OptimizerBase::ShowCurrentResults() {
 }
 
 // Optimize (optimizers.cu:183-410, :734-970): DIRECT stage over its entries, then each
-// enabled BOBYQA round over its entries, minimising -log(LC2); each stage starts from
-// and writes back to the running optimum.
+// enabled BOBYQA round over its entries, minimising -log(LC2) (or -NMI for patchSize 0);
+// each stage starts from and writes back to the running optimum.
 void OptimizerBase::Optimize() {
     if (!interp_) throw std::runtime_error("optimizer is not linked to an interpolator");
-    if (interp_->patch_size() != cfg_.patch_size) interp_->GetOutputPath();
+    // the optimizer's PatchSize (read at optimizers.cu:577) is what it evaluates with (:705)
+    interp_->set_patch_size(cfg_.patch_size);
+    auto sim_of = [this](double f) { return interp_->patch_size() == 0 ? -f : std::exp(-f); };
     const std::string logp = interp_->GetOutputPath().empty() ? std::string("./RegistrationInfo.log")
                                                               : interp_->GetOutputPath() + "/RegistrationInfo.log";
     m_logFileId.open(logp, std::ofstream::out);
@@ -659,11 +680,11 @@ void OptimizerBase::Optimize() {
             std::vector<double> l, u, x;
             bounds_for(l, u, x);
             lc2opt::Result r = lc2opt::direct_l(obj, l, u, cfg_.direct_maxeval);
-            if (r.code < 0) std::cout << "optimizer failed: " << r.message << std::endl;
+            if (r.code < 0) std::clog << "optimizer failed: " << r.message << std::endl;
             else {
                 assign_back(r.x);
-                SimilarityMeasure = std::exp(-r.f);
-                std::cout << "Optimization stage-DIRECT finished: similarity -> " << SimilarityMeasure
+                SimilarityMeasure = sim_of(r.f);
+                std::clog << "Optimization stage-DIRECT finished: similarity -> " << SimilarityMeasure
                           << ", accu evaluations -> " << EvalCounter << std::endl;
             }
         }
@@ -680,12 +701,12 @@ void OptimizerBase::Optimize() {
         bounds_for(l, u, x);
         lc2opt::Result r = lc2opt::qmtr(obj, x, l, u, rd.abs_tol, 100000);
         if (r.code < 0) {
-            std::cout << "optimizer failed: " << r.message << std::endl;
+            std::clog << "optimizer failed: " << r.message << std::endl;
             continue;
         }
         assign_back(r.x);
-        SimilarityMeasure = std::exp(-r.f);
-        std::cout << "Optimization stage-BOBYQA finished: similarity -> " << SimilarityMeasure
+        SimilarityMeasure = sim_of(r.f);
+        std::clog << "Optimization stage-BOBYQA finished: similarity -> " << SimilarityMeasure
                   << ", accu evaluations -> " << EvalCounter << std::endl;
     }
     m_logFileId.close();
@@ -735,7 +756,7 @@ void OptimizerBase::ExportResults() {
     const std::time_t t = std::chrono::system_clock::to_time_t(std::chrono::system_clock::now());
     out["RegistrationDateTime"] = lc2yaml::Node::scalar(std::string(std::ctime(&t)), true);
     lc2yaml::save_file(y, result_path());
-    std::cout << "Last Similarity Estimate: " << SimilarityMeasure << ", with " << EvalCounter
+    std::clog << "Last Similarity Estimate: " << SimilarityMeasure << ", with " << EvalCounter
               << " evaluations. " << std::endl;
 }
 

@@ -112,6 +112,15 @@ struct lc2e_pair {
     size_t cap_bytes = 0;
     bool use_graph = false;
     std::map<int, GraphEntry> graphs;
+    // mutual information (patchSize 0): extremes of the static image, per-pose extremes,
+    // joint histograms and results, sized for mi_cap poses of mi_bins counters
+    int* mi_ext_stat = nullptr;
+    bool mi_stat_ready = false;
+    int mi_cap = 0;
+    size_t mi_bins = 0;
+    int* mi_ext = nullptr;
+    unsigned* mi_hist = nullptr;
+    double* mi_out = nullptr;
     // kernel timing (eager path)
     bool profile = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -199,6 +208,18 @@ void free_batch(lc2e_pair* p) {
     p->h_out = nullptr;
     p->cap = 0;
     p->part_bpp = 0;
+}
+
+void free_mi(lc2e_pair* p) {
+    lc2e_engine* e = p->e;
+    if (p->mi_ext) e->release(p->mi_ext, (size_t)p->mi_cap * 2 * sizeof(int));
+    if (p->mi_hist) e->release(p->mi_hist, (size_t)p->mi_cap * p->mi_bins * sizeof(unsigned));
+    if (p->mi_out) e->release(p->mi_out, (size_t)p->mi_cap * sizeof(double));
+    p->mi_ext = nullptr;
+    p->mi_hist = nullptr;
+    p->mi_out = nullptr;
+    p->mi_cap = 0;
+    p->mi_bins = 0;
 }
 
 // Batch buffers for n poses whose launches use up to bpp work-groups per pose.
@@ -371,6 +392,13 @@ int lc2e_volume_upload(lc2e_engine* e, const void* host, lc2e_dtype dtype, const
     else if (storage == LC2E_F16) f16 = 1;
     else if (storage == LC2E_F32) f16 = 0;
     else return fail(LC2E_ERR_INVALID, "storage must be F16, F32 or AUTO");
+    if (f16 && dtype == LC2E_F32) {
+        // half storage must hold every value finitely: the samplers rely on finite voxels
+        const float* h = static_cast<const float*>(host);
+        for (long long i = 0; i < n; ++i)
+            if (!(std::fabs(h[i]) <= 65504.0f))
+                return fail(LC2E_ERR_INVALID, "value %g at %lld does not fit half storage", h[i], i);
+    }
     auto* v = new lc2e_volume();
     v->e = e;
     v->g = *geom;
@@ -502,6 +530,8 @@ void lc2e_pair_destroy(lc2e_pair* p) {
     if (!p) return;
     (void)hipStreamSynchronize(p->e->stream);
     free_batch(p);
+    free_mi(p);
+    if (p->mi_ext_stat) p->e->release(p->mi_ext_stat, 2 * sizeof(int));
     if (p->ev0) (void)hipEventDestroy(p->ev0);
     if (p->ev1) (void)hipEventDestroy(p->ev1);
     p->e->release(p->stat_img, p->vbytes);
@@ -576,6 +606,58 @@ int lc2e_eval_batch_async(lc2e_pair* p, int n, const float* d_mats, double* d_ou
                                    p->info.high_res_is_moving, n, st));
     }
     HIP_TRY(lc2e::launch_reduce(p->d_partials, t.bpp, n, d_out, st));
+    return LC2E_OK;
+}
+
+int lc2e_eval_mi_batch(lc2e_pair* p, int n, const float* mats, int bins_us, int bins_hr,
+                       double* nmi) {
+    if (!p || !mats || !nmi || n < 0) return fail(LC2E_ERR_INVALID, "invalid argument");
+    if (n == 0) return LC2E_OK;
+    if (bins_us < 2 || bins_hr < 2 || bins_us > lc2e::kMiMaxBins || bins_hr > lc2e::kMiMaxBins)
+        return fail(LC2E_ERR_INVALID, "histogram bins must lie in [2, %d]", lc2e::kMiMaxBins);
+    HIP_TRY(hipSetDevice(p->e->device));
+    lc2e_engine* e = p->e;
+    hipStream_t st = e->stream;
+    const int mpp = p->mats_per_pose();
+    const size_t nb = (size_t)bins_us * bins_hr;
+    int rc;
+    if (!p->mi_ext_stat) {
+        if ((rc = e->alloc((void**)&p->mi_ext_stat, 2 * sizeof(int))) < 0) return rc;
+        p->mi_stat_ready = false;
+    }
+    const int chunk_max = 64;
+    for (int base = 0; base < n; base += chunk_max) {
+        const int m = (n - base) < chunk_max ? (n - base) : chunk_max;
+        const Tiling t = tiling_for(p, m);
+        if ((rc = ensure_capacity(p, m, t.bpp)) < 0) return rc;
+        if (m > p->mi_cap || nb != p->mi_bins) {
+            free_mi(p);
+            const int cap = m > 8 ? chunk_max : 8;
+            if ((rc = e->alloc((void**)&p->mi_ext, (size_t)cap * 2 * sizeof(int))) < 0) return rc;
+            p->mi_cap = cap;
+            p->mi_bins = nb;
+            if ((rc = e->alloc((void**)&p->mi_hist, (size_t)cap * nb * sizeof(unsigned))) < 0 ||
+                (rc = e->alloc((void**)&p->mi_out, (size_t)cap * sizeof(double))) < 0) {
+                free_mi(p);
+                return rc;
+            }
+        }
+        std::memcpy(p->h_mats, mats + (size_t)base * mpp * 12, (size_t)m * mpp * 12 * sizeof(float));
+        HIP_TRY(hipMemcpyAsync(p->d_mats, p->h_mats, (size_t)m * mpp * 12 * sizeof(float),
+                               hipMemcpyHostToDevice, st));
+        lc2e::FusedArgs a = make_args(p, p->d_mats, p->d_partials, t);
+        // the static image is U when the moving image is the high-resolution one
+        const int us_static = p->info.high_res_is_moving;
+        HIP_TRY(lc2e::launch_mi(a, p->info.grid_ndim, p->info.src_ndim, us_static, m, bins_us,
+                                bins_hr, p->stat_img,
+                                p->mi_ext_stat, p->mi_stat_ready ? 1 : 0, p->mi_ext, p->mi_hist,
+                                p->mi_out, st));
+        p->mi_stat_ready = true;
+        HIP_TRY(hipMemcpyAsync(p->h_out, p->mi_out, (size_t)m * sizeof(double),
+                               hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (int i = 0; i < m; ++i) nmi[base + i] = p->h_out[i];
+    }
     return LC2E_OK;
 }
 

@@ -129,14 +129,17 @@ __device__ __forceinline__ float sample3(const T* __restrict__ src, const SGeom&
 
 // The same trilinear sample split in two for software pipelining: issue3() computes the
 // voxel coordinates and starts the four paired loads, finish3() does the lerps once the
-// data has arrived, so a whole z-step of work can run in between. Branch-free: a sample
-// outside the source (or a position outside the grid, inside == false) still loads from
-// coordinates clamped into the volume and is zeroed in finish3(); inside the volume the
-// clamp changes nothing, so the result is the reference's bit for bit.
+// data has arrived, so a whole z-step of work can run in between. Nothing but the loaded
+// values and the fractions is carried across the step (no per-lane flags):
+//  * a sample outside the source, or a position outside the grid (inside == false),
+//    loads from the volume's zeroed tail, so its lerps give exactly 0;
+//  * on an integer x the reference reads x0 twice (ceil == floor); here rx == 0 then, and
+//    fmaf(0, yr - yl, yl) == yl for the finite values a resident volume holds (f16
+//    storage rejects values beyond its range), up to the sign of a zero.
+// Inside the volume the clamps change nothing: the result is the reference's.
 struct Pending3 {
     float a0, a1, b0, b1, c0, c1, d0, d1;
     float rx, ry, rz;
-    bool ok, xs;
 };
 
 template <typename T>
@@ -166,35 +169,35 @@ __device__ __forceinline__ void issue3(const T* __restrict__ src, const SGeom& s
     float vx, vy, vz;
     to_voxel(s, px, py, pz, vx, vy, vz);
     const float xm = (float)(s.X - 1), ym = (float)(s.Y - 1), zm = (float)(s.Z - 1);
-    p.ok = inside && vx >= 0.0f && vx <= xm && vy >= 0.0f && vy <= ym && vz >= 0.0f && vz <= zm;
+    const bool ok =
+        inside && vx >= 0.0f && vx <= xm && vy >= 0.0f && vy <= ym && vz >= 0.0f && vz <= zm;
     vx = fminf(fmaxf(vx, 0.0f), xm);
     vy = fminf(fmaxf(vy, 0.0f), ym);
     vz = fminf(fmaxf(vz, 0.0f), zm);
     const float fx = floorf(vx), fy = floorf(vy), fz = floorf(vz);
     const unsigned x0 = (unsigned)fx, y0 = (unsigned)fy, z0 = (unsigned)fz;
     const unsigned y1 = (unsigned)ceilf(vy), z1 = (unsigned)ceilf(vz);
-    p.xs = (fx == vx);
     p.rx = vx - fx;
     p.ry = vy - fy;
     p.rz = vz - fz;
     const unsigned X = (unsigned)s.X, sxy = (unsigned)(s.X * s.Y);
-    ld2_off(src, x0 + y0 * X + z0 * sxy, p.a0, p.a1);
-    ld2_off(src, x0 + y0 * X + z1 * sxy, p.b0, p.b1);
-    ld2_off(src, x0 + y1 * X + z0 * sxy, p.c0, p.c1);
-    ld2_off(src, x0 + y1 * X + z1 * sxy, p.d0, p.d1);
+    const unsigned tail = sxy * (unsigned)s.Z;  // first element of the zeroed tail
+    const unsigned r00 = x0 + y0 * X + z0 * sxy, r01 = x0 + y0 * X + z1 * sxy;
+    const unsigned r10 = x0 + y1 * X + z0 * sxy, r11 = x0 + y1 * X + z1 * sxy;
+    ld2_off(src, ok ? r00 : tail, p.a0, p.a1);
+    ld2_off(src, ok ? r01 : tail, p.b0, p.b1);
+    ld2_off(src, ok ? r10 : tail, p.c0, p.c1);
+    ld2_off(src, ok ? r11 : tail, p.d0, p.d1);
 }
 
 __device__ __forceinline__ float finish3(const Pending3& p) {
-    // on an integer x the reference reads x0 twice (ceil == floor)
-    const float a1 = p.xs ? p.a0 : p.a1, b1 = p.xs ? p.b0 : p.b1;
-    const float c1 = p.xs ? p.c0 : p.c1, d1 = p.xs ? p.d0 : p.d1;
     const float lu = fmaf(p.rz, p.b0 - p.a0, p.a0);
-    const float ru = fmaf(p.rz, b1 - a1, a1);
+    const float ru = fmaf(p.rz, p.b1 - p.a1, p.a1);
     const float lb = fmaf(p.rz, p.d0 - p.c0, p.c0);
-    const float rb = fmaf(p.rz, d1 - c1, c1);
+    const float rb = fmaf(p.rz, p.d1 - p.c1, p.c1);
     const float yl = fmaf(p.ry, lb - lu, lu);
     const float yr = fmaf(p.ry, rb - ru, ru);
-    return p.ok ? fmaf(p.rx, yr - yl, yl) : 0.0f;
+    return fmaf(p.rx, yr - yl, yl);
 }
 
 // Interpolate_2Dkernel sample (interpolator.cu:186-230): bilinear, y then x. The 2x3
@@ -909,6 +912,190 @@ __global__ void k_warp_moving(FusedArgs a, int grid3, int src3, float* out) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Normalized mutual information, the reference's patchSize == 0 metric
+// (GetMutualInformationSimilarityMetric, cu_lc2_3d.cu:1196-1301; histogram kernels
+// :868-923): NMI = (H_U + H_I) / H_joint over the whole virtual grid, with bins spread
+// between each image's minimum and maximum.
+// ---------------------------------------------------------------------------
+
+// Floats as ints whose signed order is the float order (for atomic min / max).
+__device__ __forceinline__ int ford(float f) {
+    const int i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7fffffff;
+}
+__device__ __forceinline__ float fdec(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7fffffff); }
+
+// The moving image warped at virtual voxel i of pose `pose` (as k_warp_moving).
+template <typename T>
+__device__ __forceinline__ float warped_at(const FusedArgs& a, int grid3, int src3, int pose,
+                                           long long i) {
+    const VGeom& v = a.v;
+    const int x = (int)(i % v.X), y = (int)((i / v.X) % v.Y), z = (int)(i / ((long long)v.X * v.Y));
+    const float* mp = a.mats + ((size_t)pose * a.mats_per_pose +
+                                (grid3 ? 0 : (a.mats_per_pose > 1 ? z : 0))) * 12;
+    float m[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) m[k] = mp[k];
+    const float lx = fmaf((float)x, v.sx, v.ox), ly = fmaf((float)y, v.sy, v.oy);
+    const float lz = grid3 ? fmaf((float)z, v.sz, v.oz) : v.oz;
+    const T* src = static_cast<const T*>(a.src);
+    if (src3) return sample3(src, a.s, m, lx, ly, lz);
+    return sample2(src, a.s, m, lx, ly);
+}
+
+template <int NTHR>
+__device__ __forceinline__ void block_minmax(float mn, float mx, int* s_red, int* dst) {
+    for (int off = 32; off > 0; off >>= 1) {
+        mn = fminf(mn, __shfl_down(mn, off, 64));
+        mx = fmaxf(mx, __shfl_down(mx, off, 64));
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        s_red[2 * wid] = ford(mn);
+        s_red[2 * wid + 1] = ford(mx);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int a = s_red[0], b = s_red[1];
+        for (int w = 1; w < NTHR / 64; ++w) {
+            a = min(a, s_red[2 * w]);
+            b = max(b, s_red[2 * w + 1]);
+        }
+        atomicMin(&dst[0], a);
+        atomicMax(&dst[1], b);
+    }
+}
+
+// Extremes of the warped moving image per pose (blockIdx.y): ext[pose] = {min, max}.
+template <typename T>
+__global__ __launch_bounds__(256) void k_mi_extrema(FusedArgs a, int grid3, int src3, int* ext) {
+    __shared__ int s_red[8];
+    const int pose = blockIdx.y;
+    const long long n = (long long)a.v.X * a.v.Y * a.v.Z;
+    float mn = INFINITY, mx = -INFINITY;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const float v = warped_at<T>(a, grid3, src3, pose, i);
+        mn = fminf(mn, v);
+        mx = fmaxf(mx, v);
+    }
+    block_minmax<256>(mn, mx, s_red, ext + 2 * pose);
+}
+
+// Extremes of a resident image.
+__global__ __launch_bounds__(256) void k_mi_extrema_img(const float* img, long long n, int* ext) {
+    __shared__ int s_red[8];
+    float mn = INFINITY, mx = -INFINITY;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        mn = fminf(mn, img[i]);
+        mx = fmaxf(mx, img[i]);
+    }
+    block_minmax<256>(mn, mx, s_red, ext);
+}
+
+__global__ void k_mi_init(int* ext, int n_pairs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_pairs) {
+        ext[2 * i] = 0x7fffffff;
+        ext[2 * i + 1] = (int)0x80000000;
+    }
+}
+
+// cu_lc2_3d.cu:882-887: bin of v; the scaling is formed in double as there, roundf()
+// rounds its float conversion.
+__device__ __forceinline__ int mi_bin(float v, float mn, float mx, int bins) {
+    const double t = (double)(v - mn) * (bins - 1.0) / (double)(mx - mn);
+    return (int)roundf((float)t);
+}
+
+// Joint histogram [pose][bin_i][bin_u] of (U, I) over chunks of at most 65535 voxels per
+// work-group: privatised in LDS as 16-bit counters packed two to a word (128 KiB for
+// 256 x 256 bins), then the non-zero bins are added to the pose's histogram.
+template <typename T>
+__global__ __launch_bounds__(512) void k_mi_joint(FusedArgs a, int grid3, int src3,
+                                                  const float* stat, int stat_is_u,
+                                                  const int* ext_mov, const int* ext_stat,
+                                                  int bins_u, int bins_i, int chunk,
+                                                  unsigned* hist) {
+    extern __shared__ unsigned s_h[];
+    const int pose = blockIdx.y;
+    const int nb = bins_u * bins_i, nw = (nb + 1) / 2;
+    for (int w = threadIdx.x; w < nw; w += 512) s_h[w] = 0u;
+    const float mnm = fdec(ext_mov[2 * pose]), mxm = fdec(ext_mov[2 * pose + 1]);
+    const float mns = fdec(ext_stat[0]), mxs = fdec(ext_stat[1]);
+    const float mnu = stat_is_u ? mns : mnm, mxu = stat_is_u ? mxs : mxm;
+    const float mni = stat_is_u ? mnm : mns, mxi = stat_is_u ? mxm : mxs;
+    __syncthreads();
+    const long long n = (long long)a.v.X * a.v.Y * a.v.Z;
+    const long long i0 = (long long)blockIdx.x * chunk;
+    const long long i1 = min(n, i0 + chunk);
+    for (long long i = i0 + threadIdx.x; i < i1; i += 512) {
+        const float vm = warped_at<T>(a, grid3, src3, pose, i);
+        const float vs = stat[i];
+        const int bu = mi_bin(stat_is_u ? vs : vm, mnu, mxu, bins_u);
+        const int bi = mi_bin(stat_is_u ? vm : vs, mni, mxi, bins_i);
+        if (bu >= 0 && bu < bins_u && bi >= 0 && bi < bins_i) {
+            const int j = bu + bins_u * bi;
+            atomicAdd(&s_h[j >> 1], (j & 1) ? 65536u : 1u);
+        }
+    }
+    __syncthreads();
+    unsigned* h = hist + (size_t)pose * nb;
+    for (int w = threadIdx.x; w < nw; w += 512) {
+        const unsigned c = s_h[w];
+        if (c & 0xffffu) atomicAdd(&h[2 * w], c & 0xffffu);
+        if ((c >> 16) && 2 * w + 1 < nb) atomicAdd(&h[2 * w + 1], c >> 16);
+    }
+}
+
+// Entropies in FP64 from the joint histogram (its marginals are the single-image
+// histograms: every finite value falls into a bin) and NMI per pose (blockIdx.x);
+// 0 when an image is constant.
+__global__ __launch_bounds__(256) void k_mi_entropy(const unsigned* hist, const int* ext_mov,
+                                                    const int* ext_stat, int bins_u, int bins_i,
+                                                    double n, double* out) {
+    __shared__ double s_red[3][256];
+    const int pose = blockIdx.x;
+    const unsigned* h = hist + (size_t)pose * bins_u * bins_i;
+    double hj = 0.0, hu = 0.0, hi = 0.0;
+    for (int j = threadIdx.x; j < bins_u * bins_i; j += 256)
+        if (h[j]) {
+            const double p = h[j] / n;
+            hj -= p * log(p);
+        }
+    for (int u = threadIdx.x; u < bins_u; u += 256) {
+        unsigned long long c = 0;
+        for (int i = 0; i < bins_i; ++i) c += h[u + bins_u * i];
+        if (c) {
+            const double p = c / n;
+            hu -= p * log(p);
+        }
+    }
+    for (int i = threadIdx.x; i < bins_i; i += 256) {
+        unsigned long long c = 0;
+        for (int u = 0; u < bins_u; ++u) c += h[u + bins_u * i];
+        if (c) {
+            const double p = c / n;
+            hi -= p * log(p);
+        }
+    }
+    s_red[0][threadIdx.x] = hj;
+    s_red[1][threadIdx.x] = hu;
+    s_red[2][threadIdx.x] = hi;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k)
+            for (int q = 0; q < 3; ++q) s_red[q][threadIdx.x] += s_red[q][threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const bool flat = !(fdec(ext_mov[2 * pose + 1]) > fdec(ext_mov[2 * pose])) ||
+                          !(fdec(ext_stat[1]) > fdec(ext_stat[0]));
+        const double HJ = s_red[0][0], HU = s_red[1][0], HI = s_red[2][0];
+        out[pose] = (flat || !(HJ > 0.0)) ? 0.0 : (HU + HI) / HJ;
+    }
+}
+
 // Static ultrasound patch statistics (weight half of cu_lc2_3d.cu:181-287 /
 // cu_lc2_2d.cu:88-180) per voxel in FP64: count of U > 1e-6, sum, sum of squares in a
 // fixed x-fastest order; mean and centred energy, or -1 where w = 0.
@@ -1153,6 +1340,52 @@ hipError_t launch_sobel(const float* img, const VGeom& v, int grid_ndim, float* 
     const long long n = (long long)v.X * v.Y * v.Z;
     hipLaunchKernelGGL(k_sobel, grid_for(n), dim3(256), 0, st, img, v, grid_ndim == 3 ? 1 : 0,
                        out);
+    return hipGetLastError();
+}
+
+hipError_t launch_mi(const FusedArgs& a, int grid_ndim, int src_ndim, int movi, int n_poses,
+                     int bins_u, int bins_i, const float* stat, int* ext_stat, int stat_ready,
+                     int* ext_mov, unsigned* hist, double* out, hipStream_t st) {
+    if (bins_u < 2 || bins_i < 2 || bins_u > kMiMaxBins || bins_i > kMiMaxBins || n_poses <= 0)
+        return hipErrorInvalidValue;
+    const long long n = (long long)a.v.X * a.v.Y * a.v.Z;
+    const int g3 = grid_ndim == 3 ? 1 : 0, s3 = src_ndim == 3 ? 1 : 0;
+    const int nb = bins_u * bins_i;
+    hipLaunchKernelGGL(k_mi_init, dim3((n_poses + 1 + 255) / 256), dim3(256), 0, st, ext_mov,
+                       n_poses);
+    if (!stat_ready) {
+        hipLaunchKernelGGL(k_mi_init, dim3(1), dim3(256), 0, st, ext_stat, 1);
+        hipLaunchKernelGGL(k_mi_extrema_img, grid_for(n), dim3(256), 0, st, stat, n, ext_stat);
+    }
+    const dim3 ge((unsigned)std::min<long long>((n + 255) / 256, 1024), (unsigned)n_poses);
+    if (a.src_f16)
+        hipLaunchKernelGGL(k_mi_extrema<__half>, ge, dim3(256), 0, st, a, g3, s3, ext_mov);
+    else
+        hipLaunchKernelGGL(k_mi_extrema<float>, ge, dim3(256), 0, st, a, g3, s3, ext_mov);
+    hipError_t e = hipMemsetAsync(hist, 0, (size_t)n_poses * nb * sizeof(unsigned), st);
+    if (e != hipSuccess) return e;
+    const int chunk = 32768;  // voxels per work-group, below the 16-bit counter limit
+    const dim3 gj((unsigned)((n + chunk - 1) / chunk), (unsigned)n_poses);
+    const size_t lds = (size_t)((nb + 1) / 2) * sizeof(unsigned);
+    static bool configured = false;
+    if (!configured) {
+        for (const void* f : {reinterpret_cast<const void*>(&k_mi_joint<float>),
+                              reinterpret_cast<const void*>(&k_mi_joint<__half>)}) {
+            e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)((kMiMaxBins * kMiMaxBins / 2) * sizeof(unsigned)));
+            if (e != hipSuccess) return e;
+        }
+        configured = true;
+    }
+    const int stat_is_u = movi ? 1 : 0;
+    if (a.src_f16)
+        hipLaunchKernelGGL(k_mi_joint<__half>, gj, dim3(512), lds, st, a, g3, s3, stat, stat_is_u,
+                           ext_mov, ext_stat, bins_u, bins_i, chunk, hist);
+    else
+        hipLaunchKernelGGL(k_mi_joint<float>, gj, dim3(512), lds, st, a, g3, s3, stat, stat_is_u,
+                           ext_mov, ext_stat, bins_u, bins_i, chunk, hist);
+    hipLaunchKernelGGL(k_mi_entropy, dim3(n_poses), dim3(256), 0, st, hist, ext_mov, ext_stat,
+                       bins_u, bins_i, (double)n, out);
     return hipGetLastError();
 }
 

@@ -90,6 +90,14 @@ hipError_t launch_ustats(const float* img, const VGeom& v, int grid_ndim, int r,
 // |Sobel| of a virtual-grid image (3D, or per frame for 2D grids), separable order.
 hipError_t launch_sobel(const float* img, const VGeom& v, int grid_ndim, float* out,
                         hipStream_t st);
+// Normalized mutual information per pose over the virtual grid (the patchSize 0 metric):
+// extremes (the static image's once, stat_ready = 0 computes them into ext_stat), the
+// joint histogram [pose][bins_i][bins_u] and FP64 entropies into out[pose].
+// ext_mov: 2 ints per pose; hist: n_poses * bins_u * bins_i counters.
+constexpr int kMiMaxBins = 256;
+hipError_t launch_mi(const FusedArgs& a, int grid_ndim, int src_ndim, int high_res_is_moving,
+                     int n_poses, int bins_u, int bins_i, const float* stat, int* ext_stat,
+                     int stat_ready, int* ext_mov, unsigned* hist, double* out, hipStream_t st);
 // Host bytes -> resident storage (u8/i16/f16/f32 -> f16/f32).
 hipError_t launch_convert(const void* d_in, int in_dtype, void* d_out, int out_f16, long long n,
                           hipStream_t st);

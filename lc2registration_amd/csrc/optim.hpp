@@ -7,10 +7,12 @@
 //                size = longest side, trisection along the longest sides in order of the
 //                sampled values. Every iteration's new centres form ONE batch.
 //   qmtr      -- a bound-constrained quadratic-model trust-region method in the BOBYQA
-//                family: per iteration a 2n-point central-difference stencil (one batch)
-//                fits gradient and diagonal curvature, a box-constrained model step and two
-//                scalings of it form a second batch; the radius halves on failure. Stops at
-//                radius < xtol_rel (in variables scaled by 0.1*(ub-lb)) or maxeval.
+//                family: per iteration a 2n-point stencil (one batch; one-sided at a bound)
+//                fits gradient and diagonal curvature; the box-constrained model step at
+//                three scalings plus two pattern moves along the last accepted step form a
+//                second batch. The radius shrinks 4x when nothing improves and doubles after
+//                a step that gains 3/4 of the model's prediction. Stops at radius
+//                < max(10 xtol_rel, 1e-9) in variables scaled by 0.1 (ub - lb), or maxeval.
 // Both call the objective with many points at once so the caller can evaluate them in
 // one fused GPU launch.
 #pragma once

@@ -73,6 +73,8 @@ def lib():
         L.lc2e_eval_batch_async.argtypes = [vp, C.c_int, vp, vp, vp]
         L.lc2e_sync.argtypes = [vp]
         L.lc2e_debug_fetch.argtypes = [vp, C.c_int, fp, vp]
+        L.lc2e_eval_mi_batch.argtypes = [vp, C.c_int, fp, C.c_int, C.c_int,
+                                         C.POINTER(C.c_double)]
         L.lc2e_pair_set_tuning.argtypes = [vp, C.c_int]
         L.lc2e_pair_set_tuning_n.argtypes = [vp, C.c_int, C.c_int]
         L.lc2e_pair_get_tuning.argtypes = [vp, C.POINTER(C.c_int)]
@@ -221,6 +223,16 @@ class Pair:
         dt = np.float64 if stage in (STAGE_W, STAGE_MEASURE) else np.float32
         out = np.zeros(self.virt_shape, dt)
         _check(lib().lc2e_debug_fetch(self.h, stage, _fptr(m), out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def eval_mi(self, mats, bins_us: int = 256, bins_hr: int = 256) -> np.ndarray:
+        """Normalized mutual information (H_US + H_HR) / H_joint for each pose."""
+        m = np.ascontiguousarray(mats, np.float32)
+        per = 12 * (self.info.n_frames if self.info.per_frame_matrices else 1)
+        n = m.size // per
+        out = np.zeros(n, np.float64)
+        _check(lib().lc2e_eval_mi_batch(self.h, n, _fptr(m.ravel()), bins_us, bins_hr,
+                                        out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 
     def set_tuning(self, zchunk: int):

@@ -699,3 +699,65 @@ int lc2o_eval(const float* fixed, const lc2o_geom* fg, const float* moving, cons
     free(g);
     return rc;
 }
+
+/* cu_lc2_3d.cu:873-923: bin of v. The product with (bins - 1.0) is formed in double, as
+ * the reference's expression does, and roundf() rounds its float conversion. */
+static int nmi_bin(float v, float mn, float mx, int bins) {
+    const double t = (double)(v - mn) * (bins - 1.0) / (double)(mx - mn);
+    return (int)roundf((float)t);
+}
+
+static double nmi_entropy(const int* h, int nb, long long n, int fp32) {
+    /* cu_lc2_helper.cuh:20-41: p = count / n, an empty bin counts as p = 1 (0 contribution) */
+    if (fp32) {
+        float acc = 0.0f;
+        for (int b = 0; b < nb; ++b) {
+            const float p = h[b] == 0 ? 1.0f : (float)h[b] / (float)n;
+            acc += p * logf(p);
+        }
+        return -(double)acc;
+    }
+    double acc = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        if (h[b] == 0) continue;
+        const double p = (double)h[b] / (double)n;
+        acc += p * log(p);
+    }
+    return -acc;
+}
+
+double lc2o_nmi(const float* U, const float* I, long long n, int bins_u, int bins_i,
+                int fp32_entropy, int* hist_u, int* hist_i, int* hist_joint) {
+    /* cu_lc2_3d.cu:1196-1201: extremes over the whole grid */
+    float mnu = U[0], mxu = U[0], mni = I[0], mxi = I[0];
+    for (long long k = 1; k < n; ++k) {
+        if (U[k] < mnu) mnu = U[k];
+        if (U[k] > mxu) mxu = U[k];
+        if (I[k] < mni) mni = I[k];
+        if (I[k] > mxi) mxi = I[k];
+    }
+    /* a constant image has no bin width (the reference divides by zero): degenerate */
+    if (!(mxu > mnu) || !(mxi > mni)) return 0.0;
+    int* hu = (int*)calloc((size_t)bins_u, sizeof(int));
+    int* hi = (int*)calloc((size_t)bins_i, sizeof(int));
+    int* hj = (int*)calloc((size_t)bins_u * bins_i, sizeof(int));
+    for (long long k = 0; k < n; ++k) {
+        const int bu = nmi_bin(U[k], mnu, mxu, bins_u);
+        const int bi = nmi_bin(I[k], mni, mxi, bins_i);
+        const int oku = bu >= 0 && bu <= bins_u - 1, oki = bi >= 0 && bi <= bins_i - 1;
+        if (oku) hu[bu]++;
+        if (oki) hi[bi]++;
+        if (oku && oki) hj[bu + bins_u * bi]++;
+    }
+    const double h_u = nmi_entropy(hu, bins_u, n, fp32_entropy);
+    const double h_i = nmi_entropy(hi, bins_i, n, fp32_entropy);
+    const double h_j = nmi_entropy(hj, bins_u * bins_i, n, fp32_entropy);
+    if (hist_u) memcpy(hist_u, hu, sizeof(int) * bins_u);
+    if (hist_i) memcpy(hist_i, hi, sizeof(int) * bins_i);
+    if (hist_joint) memcpy(hist_joint, hj, sizeof(int) * (size_t)bins_u * bins_i);
+    free(hu);
+    free(hi);
+    free(hj);
+    if (fp32_entropy) return (double)(((float)h_u + (float)h_i) / (float)h_j);
+    return (h_u + h_i) / h_j;
+}

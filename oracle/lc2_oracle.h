@@ -84,6 +84,18 @@ int lc2o_eval(const float* fixed, const lc2o_geom* fixed_geom, const float* movi
               int high_res_is_moving, int r, const float m12[12], int solver_cg,
               int sobel_order, lc2o_result* out);
 
+/* Normalized mutual information of the patchSize == 0 path
+ * (LC2_3D_class::GetMutualInformationSimilarityMetric, cu_lc2_3d.cu:1196-1301, kernels
+ * computeHistogram / computeJointHistogram :868-923, functors cu_lc2_helper.cuh:20-41):
+ * bins from min/max of each image over the whole grid, index
+ * roundf((v - min) * (bins - 1.0) / (max - min)), p = count / n (empty bins count as
+ * p = 1), H = -sum p ln p; NMI = (H_U + H_I) / H_joint, joint index iU + binsU * iI.
+ * hist_u / hist_i / hist_joint (optional) receive the integer histograms.
+ * fp32_entropy: 1 sums p ln p in FP32 in bin order (the reference's element type; its
+ * thrust::reduce order is unpinned), 0 sums in FP64. A constant image returns 0. */
+double lc2o_nmi(const float* U, const float* I, long long n, int bins_u, int bins_i,
+                int fp32_entropy, int* hist_u, int* hist_i, int* hist_joint);
+
 int lc2o_num_threads(void);
 
 #ifdef __cplusplus

@@ -64,6 +64,8 @@ def lib():
         L.lc2o_eval.argtypes = [fp, gp, fp, gp, gp, C.c_int, C.c_int, C.c_int, C.c_int, fp,
                                 C.c_int, C.c_int, C.POINTER(Result)]
         L.lc2o_num_threads.restype = C.c_int
+        L.lc2o_nmi.argtypes = [fp, fp, C.c_longlong, C.c_int, C.c_int, C.c_int, ip, ip, ip]
+        L.lc2o_nmi.restype = C.c_double
         _lib = L
     return _lib
 
@@ -219,6 +221,21 @@ def evaluate(fixed, fixed_geom: Geom, moving, moving_geom: Geom, virt: Geom, gri
     if rc:
         raise ValueError("lc2o_eval failed")
     return _res(res)
+
+
+def nmi(U, I, bins_u: int = 256, bins_i: int = 256, fp32: bool = True, hists: bool = False):
+    """Normalized mutual information of the patchSize 0 path ((H_U + H_I) / H_joint) of
+    two images of equal size; with hists also the U, I and joint histograms."""
+    U, I = _f32(U).ravel(), _f32(I).ravel()
+    if U.size != I.size:
+        raise ValueError("images differ in size")
+    hu = np.zeros(bins_u, np.int32)
+    hi = np.zeros(bins_i, np.int32)
+    hj = np.zeros((bins_i, bins_u), np.int32)
+    ip = C.POINTER(C.c_int)
+    v = lib().lc2o_nmi(_f(U), _f(I), U.size, bins_u, bins_i, int(fp32), hu.ctypes.data_as(ip),
+                       hi.ctypes.data_as(ip), hj.ctypes.data_as(ip))
+    return (v, hu, hi, hj) if hists else v
 
 
 def num_threads() -> int:
