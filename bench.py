@@ -175,7 +175,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--poses", type=int, default=32, help="candidate poses per fused launch")
-    ap.add_argument("--zchunk", type=int, default=0, help="0: autotune (cached)")
+    ap.add_argument("--zchunk", type=int, default=0,
+                    help="z planes per work-group (0: the engine's launch model)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time the z-chunk candidates first (cached per shape)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds per CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip single-pose / fp16 figures")
@@ -209,7 +212,7 @@ def main():
     P = len(mats)
     if args.zchunk:
         pair.set_tuning_n(P, args.zchunk)
-    else:
+    elif args.autotune:
         from lc2registration_amd import autotune
         zc = autotune.tune(pair, mats)
         log(f"autotuned z-chunk {zc}")

@@ -607,7 +607,6 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
                 const float v = finish3(pend[j]);
                 if (i < NI) s_in[i] = v;
             }
-            if (s + 1 < nsteps) issue_plane(zw0 + s + 1);
             float cur0[PPT], cur1[PPT];
 #pragma unroll
             for (int j = 0; j < PPT; ++j) {
@@ -667,6 +666,8 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
                 }
             }
             __syncthreads();  // s_in reads done; products visible
+            // the next plane's gathers fly during the box sums and the combine
+            if (s + 1 < nsteps) issue_plane(zw0 + s + 1);
             if (q < 0) return;  // block-uniform: Sobel ring warm-up only
             // 3. in-plane box sums
             box_xy<NC, CNT, TX, TY, NTHR, W>(s_prod, s_cnt, s_xp, s_xcnt, s_prod, s_cnt, PS, PY,
