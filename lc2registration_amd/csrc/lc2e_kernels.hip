@@ -386,17 +386,17 @@ __device__ __forceinline__ void store_products(double* s_prod, int* s_cnt, int n
     }
 }
 
-// Window sums of width W over a run of 16 outputs: out[o] = in[o] + ... + in[o + W - 1],
-// o = 0..15, inputs in[0 .. 14 + W] (elements `stride` apart). van Herk / Gil-Werman
+// Window sums of width W over a run of NO outputs: out[o] = in[o] + ... + in[o + W - 1],
+// o = 0..NO-1, inputs in[0 .. NO + W - 2] (elements `stride` apart). van Herk / Gil-Werman
 // form: the run is cut into blocks of W inputs and a window is the suffix of its block
 // plus the prefix of the next one, so a partial sum never holds a value from outside the
 // window. (A sliding sum that adds the entering and subtracts the leaving value keeps the
 // rounding residue of every large value that passed through it, which swamps windows of
 // small variance further along the run.) Each input is read once.
-template <int W, typename V>
-__device__ __forceinline__ void window16(const V* in, int stride, V* out, int ostride) {
-    constexpr int NB = (16 + W - 1) / W;  // blocks that hold outputs
-    constexpr int NIN = 15 + W;           // inputs of the run
+template <int W, typename V, int NO = 16>
+__device__ __forceinline__ void window_run(const V* in, int stride, V* out, int ostride) {
+    constexpr int NB = (NO + W - 1) / W;  // blocks that hold outputs
+    constexpr int NIN = NO - 1 + W;       // inputs of the run
     V raw[W];
 #pragma unroll
     for (int j = 0; j < W; ++j) raw[j] = in[j * stride];
@@ -410,7 +410,7 @@ __device__ __forceinline__ void window16(const V* in, int stride, V* out, int os
 #pragma unroll
         for (int j = 0; j < W; ++j) {
             const int o = b * W + j;
-            if (o < 16) out[o * ostride] = j == 0 ? suf[0] : suf[j] + pre;
+            if (o < NO) out[o * ostride] = j == 0 ? suf[0] : suf[j] + pre;
             const int t = (b + 1) * W + j;
             if (t < NIN) {
                 raw[j] = in[t * stride];
@@ -423,35 +423,37 @@ __device__ __forceinline__ void window16(const V* in, int stride, V* out, int os
 // Box sums of width W along x then y, for NC FP64 channels and (CNT) one integer count
 // channel, over a region of py rows (row stride ps >= TX + W - 1) into TX x TY outputs:
 //   s_prod[c][py][ps] -> s_xp[c][py][TX + 1] -> s_out[c][TY][TX]   (s_out may alias s_prod).
-// Rows (x pass) and columns (y pass) are cut into runs of 16 outputs, one task each;
-// neighbouring lanes take neighbouring rows (x pass) or columns (y pass).
+// Rows (x pass) are cut into runs of 16 outputs and columns (y pass) into runs of 8, one
+// task each (runs of 8 put twice as many threads to work in the y pass); neighbouring
+// lanes take neighbouring rows (x pass) or columns (y pass).
 template <int NC, bool CNT, int TX, int TY, int NTHR, int W>
 __device__ __forceinline__ void box_xy(const double* s_prod, const int* s_cnt, double* s_xp,
                                        int* s_xcnt, double* s_out, int* s_ocnt, int ps, int py,
                                        int tid) {
     static_assert(TX % 16 == 0 && TY % 16 == 0, "tile edges are multiples of 16");
     constexpr int NT = NC + (CNT ? 1 : 0);
-    constexpr int SX = TX / 16, SY = TY / 16;
+    constexpr int SX = TX / 16;
     constexpr int XS = odd_stride(TX + 1);
     const int np = ps * py;
     for (int t = tid; t < NT * py * SX; t += NTHR) {
         const int row = t % py, rest = t / py;
         const int seg = rest % SX, c = rest / SX;
         if (c < NC)
-            window16<W>(s_prod + c * np + row * ps + seg * 16, 1,
+            window_run<W>(s_prod + c * np + row * ps + seg * 16, 1,
                         s_xp + (c * py + row) * XS + seg * 16, 1);
         else
-            window16<W>(s_cnt + row * ps + seg * 16, 1, s_xcnt + row * XS + seg * 16, 1);
+            window_run<W>(s_cnt + row * ps + seg * 16, 1, s_xcnt + row * XS + seg * 16, 1);
     }
     __syncthreads();
-    for (int t = tid; t < NT * TX * SY; t += NTHR) {
+    constexpr int SY8 = TY / 8;  // y runs of 8 outputs
+    for (int t = tid; t < NT * TX * SY8; t += NTHR) {
         const int col = t % TX, rest = t / TX;
-        const int seg = rest % SY, c = rest / SY;
+        const int seg = rest % SY8, c = rest / SY8;
         if (c < NC)
-            window16<W>(s_xp + (c * py + seg * 16) * XS + col, XS,
-                        s_out + c * TX * TY + seg * 16 * TX + col, TX);
+            window_run<W, double, 8>(s_xp + (c * py + seg * 8) * XS + col, XS,
+                                   s_out + c * TX * TY + seg * 8 * TX + col, TX);
         else
-            window16<W>(s_xcnt + seg * 16 * XS + col, XS, s_ocnt + seg * 16 * TX + col, TX);
+            window_run<W, int, 8>(s_xcnt + seg * 8 * XS + col, XS, s_ocnt + seg * 8 * TX + col, TX);
     }
     __syncthreads();
 }
