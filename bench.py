@@ -157,6 +157,170 @@ def registration_latency_figures(synth, n_runs=8):
     return out
 
 
+def _serve_closed_loop(S, eng, pair, mats, max_poses, slots, duration_s, clients, seed):
+    """`clients` threads, each submitting 1-8 poses of `mats` and waiting, for duration_s."""
+    import threading
+    srv = S.Server(eng, max_poses=max_poses, window_us=0.0, slots=slots)
+    lat, stop = [[] for _ in range(clients)], time.perf_counter() + duration_s
+
+    def client(c):
+        rng = np.random.default_rng(seed + c)
+        while time.perf_counter() < stop:
+            idx = rng.integers(0, len(mats), int(rng.integers(1, 9)))
+            t = time.perf_counter()
+            srv.wait(srv.submit(pair, mats[idx]))
+            lat[c].append(time.perf_counter() - t)
+
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=client, args=(c,)) for c in range(clients)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    dt = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    ms = np.array([x for l in lat for x in l]) * 1e3
+    return {"poses_per_s": st["poses"] / dt, "p50_ms": float(np.percentile(ms, 50)),
+            "p99_ms": float(np.percentile(ms, 99)),
+            "poses_per_launch": st["poses"] / max(st["launches"], 1)}
+
+
+def _serve_poisson(S, eng, classes, lam, max_poses, slots, duration_s, seed):
+    """Open-loop Poisson arrivals at lam requests/s; each request draws its class by weight
+    and 1-8 candidate poses of that class's pair. Returns per-class p50 / p99 latency (from
+    arrival to result) and the served poses/s."""
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(seed)
+    n = max(int(lam * duration_s), 1)
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, n))
+    weights = np.array([c[3] for c in classes], np.float64)
+    kind = rng.choice(len(classes), n, p=weights / weights.sum())
+    srv = S.Server(eng, max_poses=max_poses, window_us=0.0, slots=slots)
+    lat = [[] for _ in classes]
+
+    def collect(ticket, c, t_arr):
+        srv.wait(ticket)
+        lat[c].append(time.perf_counter() - t_arr)
+
+    with ThreadPoolExecutor(max_workers=32) as pool:
+        t0 = time.perf_counter()
+        for a, c in zip(arrivals, kind):
+            now = time.perf_counter() - t0
+            if a > now:
+                time.sleep(a - now)
+            _, pair, mats, _ = classes[c]
+            idx = rng.integers(0, len(mats), int(rng.integers(1, 9)))
+            t_arr = t0 + a
+            pool.submit(collect, srv.submit(pair, mats[idx]), c, t_arr)
+    dt = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    out = {"served_poses_per_s": st["poses"] / dt,
+           "poses_per_launch": st["poses"] / max(st["launches"], 1),
+           "max_launches_in_flight": st["max_in_flight"]}
+    for (name, _, _, _), l in zip(classes, lat):
+        ms = np.array(l) * 1e3
+        out[name] = {"requests": len(l), "p50_ms": float(np.percentile(ms, 50)) if len(l) else None,
+                     "p99_ms": float(np.percentile(ms, 99)) if len(l) else None}
+    return out
+
+
+def serving_figures(eng, E, synth, pair3d, mats3d, duration_s=2.0):
+    """Serving loop (lc2serve.h): resident pairs of three request classes, after the mixed
+    serving workload (40% 2D 256x256 affine, 40% 2D-to-3D tracked sweeps of 32 frames
+    512x384 against MR 256x256x192 f16, 20% 3D affine 192x192x160 US against MR 256x256x192),
+    each request a candidate set of 1-8 poses. Open-loop Poisson arrivals at 20, 100 and
+    200 requests/s through continuous batching (each pair's queued requests merged into one
+    launch of up to 64 poses, up to 4 launches of different pairs on the device at once),
+    and at 200 requests/s through a plain queue (one request per launch, one launch at a
+    time); then 8 closed-loop clients on the 3D pair, in both modes."""
+    from lc2registration_amd import serve as S
+    t0 = time.time()
+    w2 = synth.config0(seed=21)
+    ws = synth.config2(seed=22, mr_dims=(256, 256, 192), n_frames=32)
+    log(f"serving workloads synthesised in {time.time() - t0:.1f}s")
+    keep = []
+    f, m = w2.fixed, w2.moving
+    fv2, mv2 = eng.upload(f.data, f.spacing, f.origin), eng.upload(m.data, m.spacing, m.origin)
+    p2 = eng.pair(fv2, mv2, w2.resampled_spacing, w2.high_res_is_moving, w2.center_overlaid,
+                  w2.patch_radius, 2)
+    f, m = ws.fixed, ws.moving
+    fvs = eng.upload(f.data, f.spacing, f.origin)
+    mvs = eng.upload(m.data, m.spacing, m.origin, E.F16)
+    ps = eng.pair(fvs, mvs, ws.resampled_spacing, True, False, ws.patch_radius, 2)
+    ps.set_per_frame(True)
+    keep += [p2, ps, fv2, mv2, fvs, mvs]
+    rng = np.random.default_rng(5)
+    m2 = np.stack([p2.pose_to_matrix(synth.perturb(w2.true_pose, rng, 2)) for _ in range(64)])
+    msw = np.stack([synth.sweep_matrices(ws.true_pose + np.r_[rng.uniform(-2, 2, 3), rng.uniform(-2, 2, 3)],
+                                         ws.extra["frames"], ws.extra["centre"]) for _ in range(64)])
+    classes = [("2d_affine_256", p2, m2, 0.4), ("sweep_32x512x384", ps, msw, 0.4),
+               ("3d_affine_192x192x160", pair3d, mats3d, 0.2)]
+    for _, p, mm, _ in classes:            # first launch of each pair and batch size out of the way
+        for k in range(1, 9):
+            p.eval_batch(mm[:k])
+    out = {"serve_poisson": {}}
+    for lam, mp, sl in ((20, 64, 4), (100, 64, 4), (200, 64, 4), (200, 1, 1)):
+        key = f"{lam}_per_s" + ("_continuous" if mp > 1 else "_queue_one_launch_at_a_time")
+        r = _serve_poisson(S, eng, classes, lam, mp, sl, duration_s, seed=lam + mp)
+        out["serve_poisson"][key] = r
+        log(f"serving Poisson {key}: " + ", ".join(
+            f"{n} p50 {r[n]['p50_ms']:.2f} / p99 {r[n]['p99_ms']:.2f} ms" for n, *_ in classes
+            if r[n]["requests"]) + f"; {r['poses_per_launch']:.1f} poses per launch")
+    for mode, mp, sl in (("continuous", 64, 4), ("queue_one_launch_at_a_time", 1, 1)):
+        r = _serve_closed_loop(S, eng, pair3d, mats3d, mp, sl, duration_s, 8, 11)
+        out[f"serve_closed_loop_3d_{mode}"] = r
+        log(f"serving closed loop 3D ({mode}): {r['poses_per_s']:.0f} poses/s, p50 "
+            f"{r['p50_ms']:.2f} ms, p99 {r['p99_ms']:.2f} ms, {r['poses_per_launch']:.1f} poses per launch")
+    for h in keep:
+        h.close()
+    return out
+
+
+def ct_precision_figures(eng, E, synth, poses=32, steps=3):
+    """CT precision workload: int16 CT 512x512x400 (0.8 mm, HU) against a 256^3 uint8
+    ultrasound (0.6 mm), 7^3 patch, 32 poses per launch. The CT stored as half (exact for
+    HU in [-2048, 2048], the engine's automatic choice) and as f32: evaluations/s, kernel
+    time and whether LC2 is bit-identical."""
+    t0 = time.time()
+    w = synth.config3()
+    log(f"CT workload synthesised in {time.time() - t0:.1f}s")
+    f, m = w.fixed, w.moving
+    fv = eng.upload(f.data, f.spacing, f.origin)
+    out, res = {}, {}
+    rng = np.random.default_rng(31)
+    qs = [w.true_pose] + [synth.perturb(w.true_pose, rng, 3) for _ in range(poses - 1)]
+    for name, storage in (("f16", E.STORE_AUTO), ("f32", E.F32)):
+        u0, _ = eng.memory()
+        mv = eng.upload(m.data, m.spacing, m.origin, storage)
+        u1, _ = eng.memory()
+        p = eng.pair(fv, mv, w.resampled_spacing, True, False, w.patch_radius, 3)
+        mats = np.stack([p.pose_to_matrix(q) for q in qs])
+        res[name] = p.eval_batch(mats)
+        eng.sync()
+        p.profile(True)
+        t = time.perf_counter()
+        for _ in range(steps):
+            p.eval_batch(mats)
+        dt = (time.perf_counter() - t) / steps
+        kms, nl = p.profile_read()
+        out[f"ct_{name}_evals_per_s"] = len(mats) / dt
+        out[f"ct_{name}_kernel_ms_per_launch"] = kms / max(nl, 1)
+        out[f"ct_{name}_volume_bytes"] = u1 - u0
+        p.close()
+        mv.close()
+    fv.close()
+    out["ct_f16_lc2_bit_identical_to_f32"] = bool(np.array_equal(res["f16"].lc2, res["f32"].lc2))
+    out["ct_f16_vs_f32_max_abs_dlc2"] = float(np.abs(res["f16"].lc2 - res["f32"].lc2).max())
+    out["ct_lc2_at_true_pose"] = float(res["f16"].lc2[0])
+    out["ct_config"] = (f"CT int16 512x512x400 (0.8 mm, HU {int(m.data.min())}..{int(m.data.max())}) "
+                        f"vs US 256^3 u8 (0.6 mm), 7^3 patch, {poses} poses per launch")
+    log(f"CT: f16 {out['ct_f16_evals_per_s']:.0f} evals/s, f32 {out['ct_f32_evals_per_s']:.0f} "
+        f"evals/s, LC2 bit-identical: {out['ct_f16_lc2_bit_identical_to_f32']}")
+    return out
+
+
 def calibrate_copy(nbytes):
     """A copy kernel of known size: under rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per
     dispatch, divided by nbytes, give the factor that turns those counters into bytes."""
@@ -278,6 +442,8 @@ def main():
         mv16.close()
         log(f"single-pose fp32 {extra['single_pose_fp32_evals_per_s']:.1f} evals/s, "
             f"batched fp16 {extra['batched_fp16_evals_per_s']:.1f} evals/s")
+        extra.update(serving_figures(eng, E, synth, pair, mats))
+        extra.update(ct_precision_figures(eng, E, synth))
         extra.update(sweep_figures(eng, E, synth))
         extra.update(registration_latency_figures(synth))
 

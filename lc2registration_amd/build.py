@@ -27,7 +27,7 @@ DEVICE_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-co
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
 
 DEVICE_SOURCES = ["lc2e_kernels.hip"]
-HOST_SOURCES = ["lc2e_engine.cpp", "facade.cpp", "yaml_lite.cpp", "optim.cpp", "image_io.cpp",
+HOST_SOURCES = ["lc2e_engine.cpp", "lc2e_serve.cpp", "facade.cpp", "yaml_lite.cpp", "optim.cpp", "image_io.cpp",
                 "lc2reg_capi.cpp"]
 CLI_SOURCES = ["main_registration.cpp"]
 
@@ -63,7 +63,7 @@ def build(verbose: bool = False) -> str:
             print(f"  {os.path.basename(j[-1])}: {t:.1f}s")
     objs = [_obj(s) for s in DEVICE_SOURCES + HOST_SOURCES if os.path.exists(os.path.join(CSRC, s))]
     lib = os.path.join(LIB_DIR, "liblc2e.so")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", lib + ".tmp", *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", lib + ".tmp", *objs])
     os.replace(lib + ".tmp", lib)
     cli_objs = [_obj(s) for s in CLI_SOURCES if os.path.exists(os.path.join(CSRC, s))]
     if cli_objs:

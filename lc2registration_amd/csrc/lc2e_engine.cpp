@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -54,19 +55,29 @@ struct lc2e_engine {
     hipStream_t stream = nullptr;
     size_t budget = 0;  // 0 = unlimited
     size_t used = 0;
+    std::mutex mu;      // guards `used`: a serving loop allocates from its own thread
 
     int alloc(void** p, size_t bytes) {
-        if (budget && used + bytes > budget)
-            return fail(LC2E_ERR_BUDGET, "HBM budget exceeded: %zu used + %zu requested > %zu",
-                        used, bytes, budget);
-        HIP_TRY(hipSetDevice(device));
-        HIP_TRY(hipMalloc(p, bytes ? bytes : 1));
-        used += bytes;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            if (budget && used + bytes > budget)
+                return fail(LC2E_ERR_BUDGET, "HBM budget exceeded: %zu used + %zu requested > %zu",
+                            used, bytes, budget);
+            used += bytes;  // reserved before the allocation, returned if it fails
+        }
+        hipError_t err = hipSetDevice(device);
+        if (err == hipSuccess) err = hipMalloc(p, bytes ? bytes : 1);
+        if (err != hipSuccess) {
+            std::lock_guard<std::mutex> lk(mu);
+            used -= bytes;
+            return fail(LC2E_ERR_HIP, "hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(err));
+        }
         return LC2E_OK;
     }
     void release(void* p, size_t bytes) {
         if (p) {
             (void)hipFree(p);
+            std::lock_guard<std::mutex> lk(mu);
             used -= bytes;
         }
     }
@@ -336,6 +347,16 @@ int run_batch(lc2e_pair* p, int n, const Tiling& t) {
 
 }  // namespace
 
+namespace lc2e {
+// Sets the calling thread's lc2e_last_error() text (the serving loop hands a worker
+// thread's message to the client that collects the request).
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+// Device memory charged to the engine's HBM budget, for the serving loop's launch slots.
+int engine_alloc(lc2e_engine* e, void** p, size_t bytes) { return e->alloc(p, bytes); }
+void engine_release(lc2e_engine* e, void* p, size_t bytes) { e->release(p, bytes); }
+int engine_device(lc2e_engine* e) { return e->device; }
+}  // namespace lc2e
+
 extern "C" {
 
 const char* lc2e_last_error(void) { return g_last_error.c_str(); }
@@ -371,6 +392,7 @@ void lc2e_destroy(lc2e_engine* e) {
 
 int lc2e_memory(lc2e_engine* e, size_t* used, size_t* budget) {
     if (!e) return fail(LC2E_ERR_INVALID, "null engine");
+    std::lock_guard<std::mutex> lk(e->mu);
     if (used) *used = e->used;
     if (budget) *budget = e->budget;
     return LC2E_OK;
@@ -388,7 +410,17 @@ int lc2e_volume_upload(lc2e_engine* e, const void* host, lc2e_dtype dtype, const
     const long long n = (long long)geom->dims[0] * geom->dims[1] * geom->dims[2];
     if (n >= (1LL << 31)) return fail(LC2E_ERR_UNSUPPORTED, "volume exceeds 2^31 voxels");
     int f16;
-    if (storage == LC2E_STORE_AUTO) f16 = (dtype == LC2E_U8 || dtype == LC2E_F16) ? 1 : 0;
+    if (storage == LC2E_STORE_AUTO) {
+        f16 = (dtype == LC2E_U8 || dtype == LC2E_F16) ? 1 : 0;
+        if (dtype == LC2E_I16) {
+            // every integer in [-2048, 2048] is exact in half precision (11-bit significand),
+            // which covers CT in Hounsfield units: half storage is then lossless
+            const short* h = static_cast<const short*>(host);
+            f16 = 1;
+            for (long long i = 0; i < n && f16; ++i)
+                if (h[i] < -2048 || h[i] > 2048) f16 = 0;
+        }
+    }
     else if (storage == LC2E_F16) f16 = 1;
     else if (storage == LC2E_F32) f16 = 0;
     else return fail(LC2E_ERR_INVALID, "storage must be F16, F32 or AUTO");

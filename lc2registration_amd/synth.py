@@ -313,3 +313,95 @@ def config2(seed=2, mr_dims=(320, 320, 256), n_frames=128, frame_dims=(512, 384)
     moving = Image(mr, tuple(mr_dims), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0))
     return Workload("sweep_2d_to_3d", fixed, moving, (frame_spacing, frame_spacing), patch_radius,
                     True, False, 2, pose, extra={"frames": frames, "centre": centre})
+
+
+def _paint_ellipsoids(vol, spacing, ellipsoids, value_of, slab=32):
+    """Paints seeded ellipsoids (centre mm, semi-axes mm, rotation, value) into vol (z, y, x)
+    in order, slab by slab along z to bound the temporaries."""
+    Z, Y, X = vol.shape
+    for c, ax, R, val in ellipsoids:
+        r = ax.max()
+        lo = np.maximum(np.floor((c - r) / spacing).astype(int), 0)
+        hi = np.minimum(np.ceil((c + r) / spacing).astype(int) + 1, [X, Y, Z])
+        if np.any(hi <= lo):
+            continue
+        for z0 in range(lo[2], hi[2], slab):
+            z1 = min(z0 + slab, hi[2])
+            zz, yy, xx = np.mgrid[z0:z1, lo[1]:hi[1], lo[0]:hi[0]].astype(np.float32)
+            d = np.stack([xx * spacing - c[0], yy * spacing - c[1], zz * spacing - c[2]], -1)
+            d = d @ R.astype(np.float32)
+            m = ((d / ax.astype(np.float32)) ** 2).sum(-1) <= 1.0
+            sub = vol[z0:z1, lo[1]:hi[1], lo[0]:hi[0]]
+            sub[m] = value_of(val, m.sum())
+
+
+def _rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+CT_AIR, CT_NOISE_HU = -1000, 20.0
+
+
+def config3(seed=3, ct_dims=(512, 512, 400), ct_spacing=0.8, us_dims=(256, 256, 256),
+            us_spacing=0.6, patch_radius=3, n_organs=12, n_bones=10):
+    """CT precision workload: an int16 CT in Hounsfield units (air -1000 outside a body
+    ellipsoid of soft tissue 40 HU, organs U[20, 80] HU, bones U[1000, 2000] HU, plus
+    N(0, 20 HU) noise, rounded) is the moving volume; the ultrasound (uint8, us_spacing mm,
+    centred in the CT) is simulated from bone-surface gradients of the CT at a seeded
+    12-DoF pose: echo from |grad CT| (bone surfaces dominate) plus a weak tissue echo,
+    acoustic shadow beyond the first bone along the depth axis (+y), Rayleigh speckle."""
+    rng = _rng(seed)
+    X, Y, Z = ct_dims
+    ext = np.array(ct_dims, np.float64) * ct_spacing
+    ct = np.full((Z, Y, X), float(CT_AIR), np.float32)
+    body = [(ext / 2, ext / 2 * np.array([0.9, 0.75, 0.95]), np.eye(3), 40.0)]
+    _paint_ellipsoids(ct, ct_spacing, body, lambda v, k: v)
+    organs, bones = [], []
+    for _ in range(n_organs):
+        c = ext / 2 + rng.uniform(-0.3, 0.3, 3) * ext
+        organs.append((c, rng.uniform(0.04, 0.12, 3) * ext.min(), _rotation(rng), rng.uniform(20, 80)))
+    for _ in range(n_bones):
+        c = ext / 2 + rng.uniform(-0.3, 0.3, 3) * ext
+        ax = np.array([rng.uniform(0.015, 0.03), rng.uniform(0.015, 0.03), rng.uniform(0.1, 0.25)])
+        bones.append((c, ax * ext.min(), _rotation(rng), rng.uniform(1000, 2000)))
+    _paint_ellipsoids(ct, ct_spacing, organs, lambda v, k: v)
+    _paint_ellipsoids(ct, ct_spacing, bones, lambda v, k: v)
+    for z0 in range(0, Z, 32):   # noise, slab by slab
+        sl = ct[z0:z0 + 32]
+        sl += rng.normal(0.0, CT_NOISE_HU, sl.shape).astype(np.float32)
+    ct_i16 = np.clip(np.rint(ct), -32768, 32767).astype(np.int16)
+    del ct
+    pose = np.concatenate([rng.uniform(-8, 8, 3), rng.uniform(-8, 8, 3),
+                           rng.uniform(0.97, 1.03, 3), rng.uniform(-0.03, 0.03, 3)])
+    ct_img = Image(ct_i16, tuple(ct_dims), (ct_spacing,) * 3, (0.0, 0.0, 0.0))
+    us_ext = np.array(us_dims) * us_spacing
+    us_origin = tuple(float(v) for v in (ext / 2 - us_ext / 2))
+    M = geo.pose_to_matrix_3d(pose, tuple(us_dims), (us_spacing,) * 3, us_origin, ct_img.dims,
+                              ct_img.spacing, ct_img.origin, (us_spacing,) * 3, center_overlaid=False)
+    UX, UY, UZ = us_dims
+    I = np.empty((UZ, UY, UX), np.float32)
+    src = ct_i16.astype(np.float32) - CT_AIR         # air = 0 outside the volume too
+    yy, xx = np.mgrid[0:UY, 0:UX].astype(np.float64)
+    for z in range(UZ):
+        L = np.stack([xx * us_spacing + us_origin[0], yy * us_spacing + us_origin[1],
+                      np.full_like(xx, z * us_spacing + us_origin[2])])
+        P = np.tensordot(M[:, :3].astype(np.float64), L, axes=1) + M[:, 3].astype(np.float64)[:, None, None]
+        I[z] = _sample(src, np.stack([P[2], P[1], P[0]]) / ct_spacing)
+    del src
+    bone = I > (1000.0 - CT_AIR) * 0.5 + 500.0          # HU above ~500
+    g = _sobel_mag(I.astype(np.float64))
+    g /= max(np.percentile(g[bone], 90) if bone.any() else g.max(), 1e-9)
+    tissue = np.where(bone, 0.05, np.where(I > 500.0, 0.15, 0.0))   # soft tissue vs air
+    us = np.minimum(g, 1.0) * 0.8 + tissue + US_OFFSET
+    shadow = np.maximum.accumulate(bone, axis=1)                   # beyond the first bone in y
+    shadow[:, 1:] = shadow[:, :-1].copy()
+    shadow[:, 0] = False
+    us = np.where(shadow & ~bone, us * 0.1, us) * _speckle(us.shape, rng)
+    fixed = Image(quantize_u8(us), tuple(us_dims), (us_spacing,) * 3, us_origin)
+    return Workload("cfg3_ct_precision", fixed, ct_img, (us_spacing,) * 3, patch_radius, True,
+                    False, 3, pose)

@@ -329,3 +329,47 @@ def test_tracked_sweep_2d_to_3d_matches_oracle(engine):
     p.close()
     fv.close()
     mv.close()
+
+
+@pytest.fixture(scope="module")
+def wct():
+    return synth.config3(ct_dims=(72, 72, 60), ct_spacing=2.0, us_dims=(40, 40, 36),
+                         us_spacing=1.5, patch_radius=2)
+
+
+def test_ct_int16_half_storage_is_lossless(engine, wct):
+    """CT in HU (int16 within [-2048, 2048]) goes to half storage under LC2E_STORE_AUTO,
+    which is exact: the warp is bit-exact against the oracle on the int16 values, LC2 is
+    bit-identical to f32 storage and within 1e-9 of the FP64 oracle. A CT value outside
+    that range keeps f32 storage."""
+    ct, us = wct.moving, wct.fixed
+    assert ct.data.dtype == np.int16 and ct.data.max() <= 2048 and ct.data.min() >= -2048
+    n = int(np.prod(ct.dims))
+    u0, _ = engine.memory()
+    mv16 = engine.upload(ct.data, ct.spacing, ct.origin)             # AUTO
+    u1, _ = engine.memory()
+    mv32 = engine.upload(ct.data, ct.spacing, ct.origin, E.F32)
+    u2, _ = engine.memory()
+    assert (u2 - u1) - (u1 - u0) == 2 * n                            # 2 B vs 4 B per voxel
+    hot = ct.data.copy()
+    hot.flat[n // 2] = 3000
+    mvh = engine.upload(hot, ct.spacing, ct.origin)                  # AUTO, out of range
+    u3, _ = engine.memory()
+    assert u3 - u2 == u2 - u1
+    mvh.close()
+    fv = engine.upload(us.data, us.spacing, us.origin)
+    args = (wct.resampled_spacing, True, False, wct.patch_radius, 3)
+    p16, p32 = engine.pair(fv, mv16, *args), engine.pair(fv, mv32, *args)
+    rng = np.random.default_rng(30)
+    poses = [wct.true_pose] + [synth.perturb(wct.true_pose, rng, 3) for _ in range(3)]
+    mats = np.stack([p16.pose_to_matrix(q) for q in poses])
+    r16, r32 = p16.eval_batch(mats), p32.eval_batch(mats)
+    assert np.array_equal(r16.lc2, r32.lc2)
+    assert np.array_equal(r16.sum_measure, r32.sum_measure)
+    for k in range(len(mats)):
+        res, I, U, G = _oracle_eval(wct, mats[k])
+        assert np.array_equal(p16.debug_fetch(E.STAGE_WARP_I, mats[k]), I)
+        assert res.n_valid > 1000
+        assert abs(r16.lc2[k] - res.lc2) < LC2_TOL, (k, r16.lc2[k], res.lc2)
+    for h in (p16, p32, fv, mv16, mv32):
+        h.close()
