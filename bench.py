@@ -343,6 +343,8 @@ def main():
                     help="z planes per work-group (0: the engine's launch model)")
     ap.add_argument("--autotune", action="store_true",
                     help="time the z-chunk candidates first (cached per shape)")
+    ap.add_argument("--mr-storage", choices=("f32", "f16"), default="f32",
+                    help="storage of the MR volume (f16 rounds its values: LC2 changes)")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds per CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip single-pose / fp16 figures")
@@ -367,7 +369,7 @@ def main():
     eng = E.Engine(0)
     f, m = w.fixed, w.moving
     fv = eng.upload(f.data, f.spacing, f.origin)
-    mv = eng.upload(m.data, m.spacing, m.origin, E.F32)
+    mv = eng.upload(m.data, m.spacing, m.origin, E.F16 if args.mr_storage == "f16" else E.F32)
     pair = eng.pair(fv, mv, w.resampled_spacing, True, False, w.patch_radius, 3)
     V = int(np.prod(pair.virt_dims))
     rng = np.random.default_rng(123)
@@ -465,8 +467,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 warp/Sobel, f64 moments/solve/sums",
         "data": "synthetic (seeded phantom + simulated ultrasound)",
-        "config": {"workload": "3D affine LC2: MR 256x256x192 f32 (1 mm, moving) vs US "
-                               "192x192x160 u8 (0.5 mm, fixed grid), patch 7^3",
+        "config": {"workload": f"3D affine LC2: MR 256x256x192 {args.mr_storage} (1 mm, moving) vs "
+                               "US 192x192x160 u8 (0.5 mm, fixed grid), patch 7^3",
                    "poses_per_step": P, "virtual_grid": list(pair.virt_dims),
                    "zchunk": zchunk_batch},
         "roofline": roofline,
