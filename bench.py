@@ -226,50 +226,72 @@ def _serve_poisson(S, eng, classes, lam, max_poses, slots, duration_s, seed):
     return out
 
 
-def serving_figures(eng, E, synth, pair3d, mats3d, duration_s=2.0):
-    """Serving loop (lc2serve.h): resident pairs of three request classes, after the mixed
-    serving workload (40% 2D 256x256 affine, 40% 2D-to-3D tracked sweeps of 32 frames
-    512x384 against MR 256x256x192 f16, 20% 3D affine 192x192x160 US against MR 256x256x192),
-    each request a candidate set of 1-8 poses. Open-loop Poisson arrivals at 20, 100 and
-    200 requests/s through continuous batching (each pair's queued requests merged into one
-    launch of up to 64 poses, up to 4 launches of different pairs on the device at once),
-    and at 200 requests/s through a plain queue (one request per launch, one launch at a
-    time); then 8 closed-loop clients on the 3D pair, in both modes."""
+def serving_figures(eng, E, synth, pair3d, mats3d, w_ct):
+    """Serving loop (lc2serve.h) on the mixed serving workload: six resident pairs in three
+    request classes, each request a candidate set of 1-8 poses of one pair:
+      40% 2D affine, 256x256 or 512x512 (u8 ultrasound warped onto an f32 image);
+      40% 2D-to-3D tracked sweeps of 32 or 64 frames 512x384 u8 against one shared
+          MR 256x256x192 stored f16;
+      20% 3D: US 192x192x160 u8 against MR 256x256x192 f32, or US 256^3 u8 against the
+          int16 CT 512x512x400 (stored f16).
+    Open-loop Poisson arrivals at 5, 20, 50, 100 and 200 requests/s through continuous
+    batching (each pair's queued requests merged into one launch of up to 64 poses, up to
+    4 launches of different pairs on the device at once), and at 200 requests/s through a
+    plain queue (one request per launch, one launch at a time); then 8 closed-loop clients
+    on the 3D MR pair, in both modes."""
     from lc2registration_amd import serve as S
     t0 = time.time()
-    w2 = synth.config0(seed=21)
-    ws = synth.config2(seed=22, mr_dims=(256, 256, 192), n_frames=32)
+    w256, w512 = synth.config0(seed=21), synth.config0(seed=23, n=512)
+    ws32 = synth.config2(seed=22, mr_dims=(256, 256, 192), n_frames=32)
+    ws64 = synth.config2(seed=22, mr_dims=(256, 256, 192), n_frames=64)   # same MR
     log(f"serving workloads synthesised in {time.time() - t0:.1f}s")
-    keep = []
-    f, m = w2.fixed, w2.moving
-    fv2, mv2 = eng.upload(f.data, f.spacing, f.origin), eng.upload(m.data, m.spacing, m.origin)
-    p2 = eng.pair(fv2, mv2, w2.resampled_spacing, w2.high_res_is_moving, w2.center_overlaid,
-                  w2.patch_radius, 2)
-    f, m = ws.fixed, ws.moving
-    fvs = eng.upload(f.data, f.spacing, f.origin)
-    mvs = eng.upload(m.data, m.spacing, m.origin, E.F16)
-    ps = eng.pair(fvs, mvs, ws.resampled_spacing, True, False, ws.patch_radius, 2)
-    ps.set_per_frame(True)
-    keep += [p2, ps, fv2, mv2, fvs, mvs]
-    rng = np.random.default_rng(5)
-    m2 = np.stack([p2.pose_to_matrix(synth.perturb(w2.true_pose, rng, 2)) for _ in range(64)])
-    msw = np.stack([synth.sweep_matrices(ws.true_pose + np.r_[rng.uniform(-2, 2, 3), rng.uniform(-2, 2, 3)],
-                                         ws.extra["frames"], ws.extra["centre"]) for _ in range(64)])
-    classes = [("2d_affine_256", p2, m2, 0.4), ("sweep_32x512x384", ps, msw, 0.4),
-               ("3d_affine_192x192x160", pair3d, mats3d, 0.2)]
+    keep, rng = [], np.random.default_rng(5)
+
+    def pair2d(w):
+        f, m = w.fixed, w.moving
+        fv, mv = eng.upload(f.data, f.spacing, f.origin), eng.upload(m.data, m.spacing, m.origin)
+        p = eng.pair(fv, mv, w.resampled_spacing, w.high_res_is_moving, w.center_overlaid,
+                     w.patch_radius, 2)
+        keep.extend([p, fv, mv])
+        return p, np.stack([p.pose_to_matrix(synth.perturb(w.true_pose, rng, 2)) for _ in range(64)])
+
+    mr = eng.upload(ws32.moving.data, ws32.moving.spacing, ws32.moving.origin, E.F16)
+
+    def sweep(w):
+        f = w.fixed
+        fv = eng.upload(f.data, f.spacing, f.origin)
+        p = eng.pair(fv, mr, w.resampled_spacing, True, False, w.patch_radius, 2)
+        p.set_per_frame(True)
+        keep.extend([p, fv])
+        return p, np.stack([synth.sweep_matrices(w.true_pose + np.r_[rng.uniform(-2, 2, 3),
+                                                                   rng.uniform(-2, 2, 3)],
+                                                 w.extra["frames"], w.extra["centre"])
+                            for _ in range(64)])
+
+    f, m = w_ct.fixed, w_ct.moving
+    fct, mct = eng.upload(f.data, f.spacing, f.origin), eng.upload(m.data, m.spacing, m.origin)
+    pct = eng.pair(fct, mct, w_ct.resampled_spacing, True, False, w_ct.patch_radius, 3)
+    keep.extend([pct, fct, mct])
+    mct_mats = np.stack([pct.pose_to_matrix(synth.perturb(w_ct.true_pose, rng, 3)) for _ in range(64)])
+    classes = [("2d_affine_256", *pair2d(w256), 0.2), ("2d_affine_512", *pair2d(w512), 0.2),
+               ("sweep_32_frames", *sweep(ws32), 0.2), ("sweep_64_frames", *sweep(ws64), 0.2),
+               ("3d_mr_us192x192x160", pair3d, mats3d, 0.1), ("3d_ct_us256", pct, mct_mats, 0.1)]
+    keep.append(mr)
+    used, _ = eng.memory()
     for _, p, mm, _ in classes:            # first launch of each pair and batch size out of the way
         for k in range(1, 9):
             p.eval_batch(mm[:k])
-    out = {"serve_poisson": {}}
-    for lam, mp, sl in ((20, 64, 4), (100, 64, 4), (200, 64, 4), (200, 1, 1)):
+    out = {"serve_resident_bytes": used, "serve_poisson": {}}
+    for lam, mp, sl in ((5, 64, 4), (20, 64, 4), (50, 64, 4), (100, 64, 4), (200, 64, 4), (200, 1, 1)):
         key = f"{lam}_per_s" + ("_continuous" if mp > 1 else "_queue_one_launch_at_a_time")
-        r = _serve_poisson(S, eng, classes, lam, mp, sl, duration_s, seed=lam + mp)
+        r = _serve_poisson(S, eng, classes, lam, mp, sl, max(2.0, 60.0 / lam), seed=lam + mp)
         out["serve_poisson"][key] = r
         log(f"serving Poisson {key}: " + ", ".join(
-            f"{n} p50 {r[n]['p50_ms']:.2f} / p99 {r[n]['p99_ms']:.2f} ms" for n, *_ in classes
-            if r[n]["requests"]) + f"; {r['poses_per_launch']:.1f} poses per launch")
+            f"{n} {r[n]['p50_ms']:.2f}/{r[n]['p99_ms']:.2f}" for n, *_ in classes
+            if r[n]["requests"]) + f" ms (p50/p99); {r['poses_per_launch']:.1f} poses per "
+            f"launch, up to {r['max_launches_in_flight']} in flight")
     for mode, mp, sl in (("continuous", 64, 4), ("queue_one_launch_at_a_time", 1, 1)):
-        r = _serve_closed_loop(S, eng, pair3d, mats3d, mp, sl, duration_s, 8, 11)
+        r = _serve_closed_loop(S, eng, pair3d, mats3d, mp, sl, 2.0, 8, 11)
         out[f"serve_closed_loop_3d_{mode}"] = r
         log(f"serving closed loop 3D ({mode}): {r['poses_per_s']:.0f} poses/s, p50 "
             f"{r['p50_ms']:.2f} ms, p99 {r['p99_ms']:.2f} ms, {r['poses_per_launch']:.1f} poses per launch")
@@ -278,14 +300,11 @@ def serving_figures(eng, E, synth, pair3d, mats3d, duration_s=2.0):
     return out
 
 
-def ct_precision_figures(eng, E, synth, poses=32, steps=3):
+def ct_precision_figures(eng, E, synth, w, poses=32, steps=3):
     """CT precision workload: int16 CT 512x512x400 (0.8 mm, HU) against a 256^3 uint8
     ultrasound (0.6 mm), 7^3 patch, 32 poses per launch. The CT stored as half (exact for
     HU in [-2048, 2048], the engine's automatic choice) and as f32: evaluations/s, kernel
     time and whether LC2 is bit-identical."""
-    t0 = time.time()
-    w = synth.config3()
-    log(f"CT workload synthesised in {time.time() - t0:.1f}s")
     f, m = w.fixed, w.moving
     fv = eng.upload(f.data, f.spacing, f.origin)
     out, res = {}, {}
@@ -444,8 +463,12 @@ def main():
         mv16.close()
         log(f"single-pose fp32 {extra['single_pose_fp32_evals_per_s']:.1f} evals/s, "
             f"batched fp16 {extra['batched_fp16_evals_per_s']:.1f} evals/s")
-        extra.update(serving_figures(eng, E, synth, pair, mats))
-        extra.update(ct_precision_figures(eng, E, synth))
+        t0 = time.time()
+        w_ct = synth.config3()
+        log(f"CT workload synthesised in {time.time() - t0:.1f}s")
+        extra.update(ct_precision_figures(eng, E, synth, w_ct))
+        extra.update(serving_figures(eng, E, synth, pair, mats, w_ct))
+        del w_ct
         extra.update(sweep_figures(eng, E, synth))
         extra.update(registration_latency_figures(synth))
 

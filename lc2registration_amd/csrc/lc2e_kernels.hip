@@ -92,6 +92,30 @@ __device__ __forceinline__ void to_voxel(const SGeom& s, float px, float py, flo
     }
 }
 
+// round(b - a) in f32 for the low / high halves of two half2 words: v_fma_mix_f32 with
+// f16 sources b and -a and the f32 constant 1.0, i.e. fma(b, 1, -a) with one rounding.
+__device__ __forceinline__ float hsub_lo(unsigned b, unsigned a) {
+    float r;
+    asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(b), "v"(a));
+    return r;
+}
+__device__ __forceinline__ float hsub_hi(unsigned b, unsigned a) {
+    float r;
+    asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel:[1,0,1] op_sel_hi:[1,0,1]"
+        : "=v"(r) : "v"(b), "v"(a));
+    return r;
+}
+__device__ __forceinline__ float hlo(unsigned w) {
+    __half2 h;
+    __builtin_memcpy(&h, &w, sizeof(h));
+    return __low2float(h);
+}
+__device__ __forceinline__ float hhi(unsigned w) {
+    __half2 h;
+    __builtin_memcpy(&h, &w, sizeof(h));
+    return __high2float(h);
+}
+
 // Interpolate_3Dkernel sample (interpolator.cu:76-174): L -> M*L -> voxel -> trilinear
 // z, y, x with floor/ceil corners; 0 outside [0, dim-1].
 template <typename T>
@@ -113,6 +137,22 @@ __device__ __forceinline__ float sample3(const T* __restrict__ src, const SGeom&
     const int sxy = s.X * s.Y;
     const int r00 = y0 * s.X + z0 * sxy, r01 = y0 * s.X + z1 * sxy;
     const int r10 = y1 * s.X + z0 * sxy, r11 = y1 * s.X + z1 * sxy;
+    if constexpr (sizeof(T) == 2) {
+        // half2 words of the two x neighbours; on an integer x (xs) rx == 0 selects x0
+        const char* base = reinterpret_cast<const char*>(src);
+        unsigned wa, wb, wc, wd;
+        __builtin_memcpy(&wa, base + (size_t)(x0 + r00) * 2, 4);
+        __builtin_memcpy(&wb, base + (size_t)(x0 + r01) * 2, 4);
+        __builtin_memcpy(&wc, base + (size_t)(x0 + r10) * 2, 4);
+        __builtin_memcpy(&wd, base + (size_t)(x0 + r11) * 2, 4);
+        const float lu = fmaf(rz, hsub_lo(wb, wa), hlo(wa));
+        const float ru = fmaf(rz, hsub_hi(wb, wa), hhi(wa));
+        const float lb = fmaf(rz, hsub_lo(wd, wc), hlo(wc));
+        const float rb = fmaf(rz, hsub_hi(wd, wc), hhi(wc));
+        const float yl = fmaf(ry, lb - lu, lu);
+        const float yr = fmaf(ry, rb - ru, ru);
+        return fmaf(rx, yr - yl, yl);
+    }
     float a0, a1, b0, b1;
     ld2(src, x0 + r00, xs, a0, a1);  // (x0, y0, z0), (x1, y0, z0)
     ld2(src, x0 + r01, xs, b0, b1);  // (x0, y0, z1), (x1, y0, z1)
@@ -137,10 +177,20 @@ __device__ __forceinline__ float sample3(const T* __restrict__ src, const SGeom&
 //    fmaf(0, yr - yl, yl) == yl for the finite values a resident volume holds (f16
 //    storage rejects values beyond its range), up to the sign of a zero.
 // Inside the volume the clamps change nothing: the result is the reference's.
+// f32 sources keep the eight corner values; half sources keep the four loaded half2 words
+// as they are, and the lerps read them through mixed-precision FMAs (exactly the f32
+// arithmetic on the converted values, without separate conversions).
+template <typename T>
 struct Pending3 {
     float a0, a1, b0, b1, c0, c1, d0, d1;
     float rx, ry, rz;
 };
+template <>
+struct Pending3<__half> {
+    unsigned a, b, c, d;  // half2: x0 in the low half, x0 + 1 in the high half
+    float rx, ry, rz;
+};
+
 
 template <typename T>
 __device__ __forceinline__ void ld2_off(const T* __restrict__ base, unsigned idx, float& a,
@@ -162,7 +212,7 @@ __device__ __forceinline__ void ld2_off(const T* __restrict__ base, unsigned idx
 
 template <typename T>
 __device__ __forceinline__ void issue3(const T* __restrict__ src, const SGeom& s, const float* m,
-                                       float lx, float ly, float lz, bool inside, Pending3& p) {
+                                       float lx, float ly, float lz, bool inside, Pending3<T>& p) {
     const float px = fmaf(m[2], lz, fmaf(m[1], ly, m[0] * lx)) + m[3];
     const float py = fmaf(m[6], lz, fmaf(m[5], ly, m[4] * lx)) + m[7];
     const float pz = fmaf(m[10], lz, fmaf(m[9], ly, m[8] * lx)) + m[11];
@@ -184,17 +234,34 @@ __device__ __forceinline__ void issue3(const T* __restrict__ src, const SGeom& s
     const unsigned tail = sxy * (unsigned)s.Z;  // first element of the zeroed tail
     const unsigned r00 = x0 + y0 * X + z0 * sxy, r01 = x0 + y0 * X + z1 * sxy;
     const unsigned r10 = x0 + y1 * X + z0 * sxy, r11 = x0 + y1 * X + z1 * sxy;
-    ld2_off(src, ok ? r00 : tail, p.a0, p.a1);
-    ld2_off(src, ok ? r01 : tail, p.b0, p.b1);
-    ld2_off(src, ok ? r10 : tail, p.c0, p.c1);
-    ld2_off(src, ok ? r11 : tail, p.d0, p.d1);
+    if constexpr (sizeof(T) == 2) {
+        const char* b = reinterpret_cast<const char*>(src);
+        __builtin_memcpy(&p.a, b + (ok ? r00 : tail) * 2u, 4);
+        __builtin_memcpy(&p.b, b + (ok ? r01 : tail) * 2u, 4);
+        __builtin_memcpy(&p.c, b + (ok ? r10 : tail) * 2u, 4);
+        __builtin_memcpy(&p.d, b + (ok ? r11 : tail) * 2u, 4);
+    } else {
+        ld2_off(src, ok ? r00 : tail, p.a0, p.a1);
+        ld2_off(src, ok ? r01 : tail, p.b0, p.b1);
+        ld2_off(src, ok ? r10 : tail, p.c0, p.c1);
+        ld2_off(src, ok ? r11 : tail, p.d0, p.d1);
+    }
 }
 
-__device__ __forceinline__ float finish3(const Pending3& p) {
-    const float lu = fmaf(p.rz, p.b0 - p.a0, p.a0);
-    const float ru = fmaf(p.rz, p.b1 - p.a1, p.a1);
-    const float lb = fmaf(p.rz, p.d0 - p.c0, p.c0);
-    const float rb = fmaf(p.rz, p.d1 - p.c1, p.c1);
+template <typename T>
+__device__ __forceinline__ float finish3(const Pending3<T>& p) {
+    float lu, ru, lb, rb;
+    if constexpr (sizeof(T) == 2) {
+        lu = fmaf(p.rz, hsub_lo(p.b, p.a), hlo(p.a));
+        ru = fmaf(p.rz, hsub_hi(p.b, p.a), hhi(p.a));
+        lb = fmaf(p.rz, hsub_lo(p.d, p.c), hlo(p.c));
+        rb = fmaf(p.rz, hsub_hi(p.d, p.c), hhi(p.c));
+    } else {
+        lu = fmaf(p.rz, p.b0 - p.a0, p.a0);
+        ru = fmaf(p.rz, p.b1 - p.a1, p.a1);
+        lb = fmaf(p.rz, p.d0 - p.c0, p.c0);
+        rb = fmaf(p.rz, p.d1 - p.c1, p.c1);
+    }
     const float yl = fmaf(p.ry, lb - lu, lu);
     const float yr = fmaf(p.ry, rb - ru, ru);
     return fmaf(p.rx, yr - yl, yl);
@@ -221,6 +288,15 @@ __device__ __forceinline__ float sample2(const T* __restrict__ src, const SGeom&
     const int x0 = (int)fx, y0 = (int)fy, y1 = (int)ceilf(vy);
     const bool xs = (fx == vx);
     const float ry = vy - fy;
+    if constexpr (sizeof(T) == 2) {
+        const char* base = reinterpret_cast<const char*>(src);
+        unsigned wa, wb;
+        __builtin_memcpy(&wa, base + (size_t)(x0 + y0 * s.X) * 2, 4);
+        __builtin_memcpy(&wb, base + (size_t)(x0 + y1 * s.X) * 2, 4);
+        const float xl = fmaf(ry, hsub_lo(wb, wa), hlo(wa));
+        const float xr = fmaf(ry, hsub_hi(wb, wa), hhi(wa));
+        return fmaf(vx - fx, xr - xl, xl);  // 0 * (xr - xl) on an integer x
+    }
     float a0, a1, b0, b1;
     ld2(src, x0 + y0 * s.X, xs, a0, a1);
     ld2(src, x0 + y1 * s.X, xs, b0, b1);
@@ -527,7 +603,7 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
     const int wofs = MOVI ? R + 1 : R;   // warped region starts at x0 - wofs, y0 - wofs
     const int nsteps = nout + 2 * R + LAG;
 
-    Pending3 pend[SPT];
+    Pending3<T> pend[SPT];
     auto issue_plane = [&](int zi) {
         const bool zin = zi >= 0 && zi < Z;
         const float lz = fmaf((float)zi, a.v.sz, a.v.oz);
