@@ -34,7 +34,6 @@ namespace lc2e {
 
 namespace {
 
-constexpr int NCH = 9;  // I, G, U, II, GG, IG, UU, UI, UG
 constexpr double kZeroThreshold = 1e-6;  // cu_lc2_helper.cuh:5
 constexpr double kTauConst = 1e-10;      // centred energy below this fraction of raw -> constant
 constexpr double kTauColl = 1e-10;       // Gram determinant below this fraction -> collinear
@@ -306,6 +305,67 @@ __device__ __forceinline__ float sample2(const T* __restrict__ src, const SGeom&
     return fmaf(rx, xr - xl, xl);
 }
 
+// sample2 split like issue3 / finish3: coordinates and the two paired loads now, the
+// lerps later. Same flag-free rules: outside the source (or inside == false) the loads
+// read the zeroed tail; on an integer x, rx == 0 selects x0.
+template <typename T>
+struct Pending2 {
+    float a0, a1, b0, b1;
+    float rx, ry;
+};
+template <>
+struct Pending2<__half> {
+    unsigned a, b;  // half2 words at (x0, y0) and (x0, y1)
+    float rx, ry;
+};
+
+template <typename T>
+__device__ __forceinline__ void issue2(const T* __restrict__ src, const SGeom& s, const float* m,
+                                       float lx, float ly, bool inside, Pending2<T>& p) {
+    const float px = fmaf(m[1], ly, m[0] * lx) + m[3];
+    const float py = fmaf(m[5], ly, m[4] * lx) + m[7];
+    float vx, vy;
+    if (s.pow2) {
+        vx = (px - s.ox) * s.rsx;
+        vy = (py - s.oy) * s.rsy;
+    } else {
+        vx = (px - s.ox) / s.sx;
+        vy = (py - s.oy) / s.sy;
+    }
+    const float xm = (float)(s.X - 1), ym = (float)(s.Y - 1);
+    const bool ok = inside && vx >= 0.0f && vx <= xm && vy >= 0.0f && vy <= ym;
+    vx = fminf(fmaxf(vx, 0.0f), xm);
+    vy = fminf(fmaxf(vy, 0.0f), ym);
+    const float fx = floorf(vx), fy = floorf(vy);
+    const unsigned x0 = (unsigned)fx, y0 = (unsigned)fy, y1 = (unsigned)ceilf(vy);
+    p.rx = vx - fx;
+    p.ry = vy - fy;
+    const unsigned X = (unsigned)s.X;
+    const unsigned tail = X * (unsigned)s.Y * (unsigned)s.Z;  // first element of the zeroed tail
+    const unsigned r0 = ok ? x0 + y0 * X : tail, r1 = ok ? x0 + y1 * X : tail;
+    if constexpr (sizeof(T) == 2) {
+        const char* b = reinterpret_cast<const char*>(src);
+        __builtin_memcpy(&p.a, b + r0 * 2u, 4);
+        __builtin_memcpy(&p.b, b + r1 * 2u, 4);
+    } else {
+        ld2_off(src, r0, p.a0, p.a1);
+        ld2_off(src, r1, p.b0, p.b1);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ float finish2(const Pending2<T>& p) {
+    float xl, xr;
+    if constexpr (sizeof(T) == 2) {
+        xl = fmaf(p.ry, hsub_lo(p.b, p.a), hlo(p.a));
+        xr = fmaf(p.ry, hsub_hi(p.b, p.a), hhi(p.a));
+    } else {
+        xl = fmaf(p.ry, p.b0 - p.a0, p.a0);
+        xr = fmaf(p.ry, p.b1 - p.a1, p.a1);
+    }
+    return fmaf(p.rx, xr - xl, xl);
+}
+
 // FP64 reciprocal and reciprocal square root: hardware estimate + two Newton steps
 // (relative error a few ulp), instead of the long IEEE division / sqrt sequences.
 __device__ __forceinline__ double rcp64(double x) {
@@ -535,7 +595,7 @@ __device__ __forceinline__ void box_xy(const double* s_prod, const int* s_cnt, d
 }
 
 // box_xy for a patch radius known only at run time (the 2D kernel), 1 <= r <= kMaxR2.
-template <int NC, bool CNT, int TX, int TY, int NTHR, int RR = 1>
+template <int NC, bool CNT, int TX, int TY, int NTHR, int RMAX = kMaxR2, int RR = 1>
 __device__ __forceinline__ void box_xy_r(int r, const double* s_prod, const int* s_cnt,
                                          double* s_xp, int* s_xcnt, double* s_out, int* s_ocnt,
                                          int ps, int py, int tid) {
@@ -543,7 +603,7 @@ __device__ __forceinline__ void box_xy_r(int r, const double* s_prod, const int*
         box_xy<NC, CNT, TX, TY, NTHR, 2 * RR + 1>(s_prod, s_cnt, s_xp, s_xcnt, s_out, s_ocnt, ps,
                                                   py, tid);
     else if constexpr (RR < kMaxR2)
-        box_xy_r<NC, CNT, TX, TY, NTHR, RR + 1>(r, s_prod, s_cnt, s_xp, s_xcnt, s_out, s_ocnt, ps,
+        box_xy_r<NC, CNT, TX, TY, NTHR, RMAX, RR + 1>(r, s_prod, s_cnt, s_xp, s_xcnt, s_out, s_ocnt, ps,
                                                 py, tid);
 }
 
@@ -804,20 +864,31 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// 2D fused kernel: one 16x16 output tile of one frame (plane) of one pose.
-// Dynamic LDS sized for the runtime patch radius.
+// 2D fused kernel: one 16x16 output tile of a run of `zchunk` consecutive frames of one
+// pose (a single image is one frame). Dynamic LDS sized for the runtime patch radius.
+// Frame f + 1's gathers (and its static image values and ultrasound statistics) are
+// issued while frame f's box sums and combine run, as in the 3D kernel's z-steps. Each
+// frame's tile sum goes to its own partial (frame * tiles + tile), so results do not
+// depend on the run length.
 // ---------------------------------------------------------------------------
-template <bool MOVI, bool SRC3D, typename T>
+template <bool MOVI, bool SRC3D, typename T, int RMAX>
 __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
     extern __shared__ double s_dyn[];
     constexpr int NC = MOVI ? 7 : 9;
     constexpr bool CNT = !MOVI;
+    // per-thread gathers / products positions at the largest radius of this variant
+    // (r <= RMAX): the pipelined gathers are held in registers
+    constexpr int SPT = ((kTile + 2 * RMAX + 2) * (kTile + 2 * RMAX + 2) + kThreads - 1) / kThreads;
+    constexpr int PPT = ((kTile + 2 * RMAX) * (kTile + 2 * RMAX) + kThreads - 1) / kThreads;
     const int R = a.r, W = 2 * R + 1, PX = kTile + 2 * R, NP = PX * PX, IXE = PX + 2;
     const int PS = odd_stride(PX);                            // products row stride
     constexpr int XS = odd_stride(kTile + 1);                 // x-pass row stride
-    const int NI = MOVI ? IXE * IXE : NP;
+    const int wex = MOVI ? IXE : PX;                          // warped region edge
+    const int wofs = MOVI ? R + 1 : R;                        // it starts at x0 - wofs
+    const int NI = wex * wex;
     // The x-pass output s_xp shares its space with the warped tile s_in, which is dead
-    // once the products are stored, before the box sums start.
+    // once the products are stored, before the box sums start; the next frame's warped
+    // tile is written only after the box sums and the combine.
     const int uni = max(2 * NC * PX * XS, NI);                // shared region, in floats
     double* s_prod = s_dyn;                                   // NC*PX*PS
     double* s_red = s_prod + NC * PX * PS;                    // 8
@@ -829,99 +900,150 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
     int tile, pose;
     if (!map_block(a.tiles, a.npose, tile, pose)) return;  // whole work-group
     const int tid = threadIdx.x;
-    const int tcol = tile % (a.ntx * a.nty), frame = tile / (a.ntx * a.nty);
+    const int cols = a.ntx * a.nty;
+    const int tcol = tile % cols, run = tile / cols;
+    const int f0 = run * a.zchunk, f1 = min(f0 + a.zchunk, a.v.Z);
     const int tx = tcol % a.ntx, ty = tcol / a.ntx;
     const int x0 = R + tx * kTile, y0 = R + ty * kTile;
     const int X = a.v.X, Y = a.v.Y;
-    const int plane = frame * X * Y;
-    float m[12];
-    {
-        const float* mp =
-            a.mats + ((size_t)pose * a.mats_per_pose + (a.mats_per_pose > 1 ? frame : 0)) * 12;
-#pragma unroll
-        for (int k = 0; k < 12; ++k) m[k] = mp[k];
-    }
     const float lz = a.v.oz;
     const T* __restrict__ src = static_cast<const T*>(a.src);
-    // 1. warp
-    if (MOVI) {
-        for (int i = tid; i < NI; i += kThreads) {
-            const int lx = i % IXE, ly = i / IXE;
-            const int gx = x0 - R - 1 + lx, gy = y0 - R - 1 + ly;
-            float val = 0.0f;
-            if (gx >= 0 && gx < X && gy >= 0 && gy < Y) {
-                const float Lx = fmaf((float)gx, a.v.sx, a.v.ox), Ly = fmaf((float)gy, a.v.sy, a.v.oy);
-                val = SRC3D ? sample3(src, a.s, m, Lx, Ly, lz) : sample2(src, a.s, m, Lx, Ly);
-            }
-            s_in[i] = val;
+    const float* mats = a.mats + (size_t)pose * a.mats_per_pose * 12;
+
+    using Pend = typename std::conditional<SRC3D, Pending3<T>, Pending2<T>>::type;
+    Pend pend[SPT];
+    auto issue_frame = [&](int frame) {
+        float m[12];
+        const float* mp = mats + (a.mats_per_pose > 1 ? frame : 0) * 12;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) m[k] = mp[k];
+#pragma unroll
+        for (int j = 0; j < SPT; ++j) {
+            if (j * kThreads >= NI) break;  // uniform
+            const int i = tid + j * kThreads;
+            const int lx = i % wex, ly = i / wex;
+            const int gx = x0 - wofs + lx, gy = y0 - wofs + ly;
+            const bool inside = i < NI && gx >= 0 && gx < X && gy >= 0 && gy < Y;
+            const float Lx = fmaf((float)gx, a.v.sx, a.v.ox), Ly = fmaf((float)gy, a.v.sy, a.v.oy);
+            if constexpr (SRC3D) issue3(src, a.s, m, Lx, Ly, lz, inside, pend[j]);
+            else issue2(src, a.s, m, Lx, Ly, inside, pend[j]);
         }
-    } else {
-        for (int i = tid; i < NI; i += kThreads) {
+    };
+    // static image values of the products positions (U if MOVI, else I and |grad I|)
+    float pre0[PPT], pre1[PPT];
+    auto issue_static = [&](int frame) {
+        const int plane = frame * X * Y;
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            if (j * kThreads >= NP) break;  // uniform
+            const int i = tid + j * kThreads;
             const int lx = i % PX, ly = i / PX;
             const int gx = x0 - R + lx, gy = y0 - R + ly;
-            float val = 0.0f;
-            if (gx < X && gy < Y) {
-                const float Lx = fmaf((float)gx, a.v.sx, a.v.ox), Ly = fmaf((float)gy, a.v.sy, a.v.oy);
-                val = SRC3D ? sample3(src, a.s, m, Lx, Ly, lz) : sample2(src, a.s, m, Lx, Ly);
-            }
-            s_in[i] = val;
+            const bool in = i < NP && gx < X && gy < Y;
+            const int vidx = plane + gy * X + gx;
+            pre0[j] = in ? a.stat_img[vidx] : 0.0f;
+            if (!MOVI) pre1[j] = in ? a.stat_grad[vidx] : 0.0f;
         }
-    }
-    __syncthreads();
-    for (int i = tid; i < NP; i += kThreads) {
-        const int lx = i % PX, ly = i / PX;
-        const int gx = x0 - R + lx, gy = y0 - R + ly;
-        const bool in = gx < X && gy < Y;
-        const int vidx = plane + gy * X + gx;
-        float I, G, U;
-        if (MOVI) {
-            // 3x3 Sobel read from the warped tile: rows smooth / difference along x
-            float sv[3], dv[3], cc = 0.0f;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const float* row = s_in + (ly + k) * IXE + lx;
-                const float l = row[0], c = row[1], r = row[2];
-                dv[k] = r - l;
-                sv[k] = (l + 2.0f * c) + r;
-                if (k == 1) cc = c;
-            }
-            const float gxv = (dv[0] + 2.0f * dv[1]) + dv[2];
-            const float gyv = sv[2] - sv[0];
-            G = sqrtf(fmaf(gyv, gyv, gxv * gxv));
-            I = cc;
-            U = in ? a.stat_img[vidx] : 0.0f;
-        } else {
-            U = s_in[i];
-            I = in ? a.stat_img[vidx] : 0.0f;
-            G = in ? a.stat_grad[vidx] : 0.0f;
-        }
-        store_products<NC>(s_prod, s_cnt, PX * PS, ly * PS + lx, I, G, U);
-    }
-    __syncthreads();
-    box_xy_r<NC, CNT, kTile, kTile, kThreads>(R, s_prod, s_cnt, s_xp, s_xcnt, s_prod, s_cnt, PS, PX,
-                                              tid);
+    };
+    // static ultrasound statistics of this thread's output pixel
     const int ox = tid % kTile, oy = tid / kTile;
-    double acc_m = 0.0, acc_w = 0.0;
-    if ((x0 + ox < X - R) && (y0 + oy < Y - R)) {
-        double S[NC];
+    const bool out_ok = (x0 + ox < X - R) && (y0 + oy < Y - R);
+    double mu_pre = 0.0, cyy_pre = -1.0;
+    auto issue_stats = [&](int frame) {
+        if (MOVI && out_ok) {
+            const int vo = frame * X * Y + (y0 + oy) * X + (x0 + ox);
+            mu_pre = a.ustat_mean[vo];
+            cyy_pre = a.ustat_cyy[vo];
+        }
+    };
+
+    issue_frame(f0);
+    issue_static(f0);
+    issue_stats(f0);
+    for (int f = f0; f < f1; ++f) {
+        // 1. the warped tile of frame f
 #pragma unroll
-        for (int c = 0; c < NC; ++c) S[c] = s_prod[c * kTile * kTile + tid];
-        double w, meas;
-        bool ok;
-        const int vo = plane + (y0 + oy) * X + (x0 + ox);
-        if (MOVI) ok = combine_static_u(S, a.ustat_mean[vo], a.ustat_cyy[vo], W * W, w, meas);
-        else ok = combine(S, s_cnt[tid], W * W, w, meas);
-        if (ok) {
-            acc_m = meas;
-            acc_w = w;
-            if (a.wmap) {
-                a.wmap[vo] = w;
-                a.mmap[vo] = meas;
+        for (int j = 0; j < SPT; ++j) {
+            if (j * kThreads >= NI) break;  // uniform
+            const int i = tid + j * kThreads;
+            float v;
+            if constexpr (SRC3D) v = finish3(pend[j]);
+            else v = finish2(pend[j]);
+            if (i < NI) s_in[i] = v;
+        }
+        float cur0[PPT], cur1[PPT];
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            cur0[j] = pre0[j];
+            cur1[j] = pre1[j];
+        }
+        const double mu = mu_pre, cyy = cyy_pre;
+        if (f + 1 < f1) {
+            issue_static(f + 1);
+            issue_stats(f + 1);
+        }
+        __syncthreads();
+        // 2. |Sobel| and the products
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            if (j * kThreads >= NP) break;  // uniform
+            const int i = tid + j * kThreads;
+            if (i < NP) {
+                const int lx = i % PX, ly = i / PX;
+                float I, G, U;
+                if (MOVI) {
+                    // 3x3 Sobel read from the warped tile: rows smooth / difference along x
+                    float sv[3], dv[3], cc = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const float* row = s_in + (ly + k) * IXE + lx;
+                        const float l = row[0], c = row[1], r = row[2];
+                        dv[k] = r - l;
+                        sv[k] = (l + 2.0f * c) + r;
+                        if (k == 1) cc = c;
+                    }
+                    const float gxv = (dv[0] + 2.0f * dv[1]) + dv[2];
+                    const float gyv = sv[2] - sv[0];
+                    G = sqrtf(fmaf(gyv, gyv, gxv * gxv));
+                    I = cc;
+                    U = cur0[j];
+                } else {
+                    U = s_in[i];
+                    I = cur0[j];
+                    G = cur1[j];
+                }
+                store_products<NC>(s_prod, s_cnt, PX * PS, ly * PS + lx, I, G, U);
             }
         }
+        __syncthreads();  // s_in reads done; products visible
+        // the next frame's gathers fly during the box sums and the combine
+        if (f + 1 < f1) issue_frame(f + 1);
+        // 3. box sums
+        box_xy_r<NC, CNT, kTile, kTile, kThreads, RMAX>(R, s_prod, s_cnt, s_xp, s_xcnt, s_prod,
+                                                        s_cnt, PS, PX, tid);
+        // 4. combine, and the frame's tile sum
+        double acc_m = 0.0, acc_w = 0.0;
+        if (out_ok) {
+            double S[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) S[c] = s_prod[c * kTile * kTile + tid];
+            double w, meas;
+            bool ok;
+            if (MOVI) ok = combine_static_u(S, mu, cyy, W * W, w, meas);
+            else ok = combine(S, s_cnt[tid], W * W, w, meas);
+            if (ok) {
+                acc_m = meas;
+                acc_w = w;
+                if (a.wmap) {
+                    const int vo = f * X * Y + (y0 + oy) * X + (x0 + ox);
+                    a.wmap[vo] = w;
+                    a.mmap[vo] = meas;
+                }
+            }
+        }
+        double* dst = a.partials + ((size_t)pose * a.blocks_per_pose + (size_t)f * cols + tcol) * 2;
+        block_reduce_store<kThreads>(acc_m, acc_w, s_red, dst);
     }
-    double* dst = a.partials + ((size_t)pose * a.blocks_per_pose + tile) * 2;
-    block_reduce_store<kThreads>(acc_m, acc_w, s_red, dst);
 }
 
 // Per-pose fixed-order reduction of the work-group partials -> (sum m, sum w, LC2)
@@ -1275,15 +1397,17 @@ __global__ void k_convert(const void* in, int in_dtype, void* out, int out_f16, 
     }
 }
 
-constexpr size_t lds_bytes_2d(int r) {
-    // the layout of k_fused2d for the 9-channel (moving ultrasound) variant, the larger one
+// The dynamic LDS layout of k_fused2d: nc FP64 channels (7 for a moving high-resolution
+// image, 9 plus an integer count for a moving ultrasound).
+constexpr size_t lds_bytes_2d(int r, bool movi) {
+    const size_t nc = movi ? 7 : 9;
     const size_t PX = kTile + 2 * r, IXE = PX + 2;
     const size_t PS = odd_stride((int)PX), XS = odd_stride(kTile + 1);
-    const size_t uni = std::max(2 * NCH * PX * XS, IXE * IXE);
-    return sizeof(double) * (NCH * PX * PS + 8) + sizeof(float) * uni +
-           sizeof(int) * (PX * PS + PX * XS);
+    const size_t uni = std::max(2 * nc * PX * XS, movi ? IXE * IXE : PX * PX);
+    return sizeof(double) * (nc * PX * PS + 8) + sizeof(float) * uni +
+           (movi ? 0 : sizeof(int) * (PX * PS + PX * XS));
 }
-static_assert(lds_bytes_2d(kMaxR2) <= 160 * 1024, "2D fused kernel exceeds the 160 KiB of LDS");
+static_assert(lds_bytes_2d(kMaxR2, false) <= 160 * 1024, "2D fused kernel exceeds the 160 KiB of LDS");
 
 template <int R, typename T>
 hipError_t launch3(const FusedArgs& a, bool movi, dim3 grid, hipStream_t st) {
@@ -1303,19 +1427,27 @@ hipError_t launch3r(const FusedArgs& a, bool movi, dim3 grid, hipStream_t st) {
     }
 }
 
-template <bool MOVI, bool SRC3D, typename T>
-hipError_t launch2(const FusedArgs& a, dim3 grid, hipStream_t st) {
-    const size_t lds = lds_bytes_2d(a.r);
-    static bool configured = false;  // allow the largest radius once; smaller ones fit under it
+template <bool MOVI, bool SRC3D, typename T, int RMAX>
+hipError_t launch2r(const FusedArgs& a, dim3 grid, hipStream_t st) {
+    const size_t lds = lds_bytes_2d(a.r, MOVI);
+    static bool configured = false;  // allow this variant's largest radius once
     if (!configured) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused2d<MOVI, SRC3D, T>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused2d<MOVI, SRC3D, T, RMAX>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)lds_bytes_2d(kMaxR2));
+                                           (int)lds_bytes_2d(RMAX, MOVI));
         if (e != hipSuccess) return e;
         configured = true;
     }
-    hipLaunchKernelGGL((k_fused2d<MOVI, SRC3D, T>), grid, dim3(kThreads), lds, st, a);
+    hipLaunchKernelGGL((k_fused2d<MOVI, SRC3D, T, RMAX>), grid, dim3(kThreads), lds, st, a);
     return hipGetLastError();
+}
+
+// Radius classes of the 2D kernel: fewer registers for the common small patches.
+template <bool MOVI, bool SRC3D, typename T>
+hipError_t launch2(const FusedArgs& a, dim3 grid, hipStream_t st) {
+    if (a.r <= 3) return launch2r<MOVI, SRC3D, T, 3>(a, grid, st);
+    if (a.r <= 6) return launch2r<MOVI, SRC3D, T, 6>(a, grid, st);
+    return launch2r<MOVI, SRC3D, T, kMaxR2>(a, grid, st);
 }
 
 template <typename T>
@@ -1353,6 +1485,7 @@ hipError_t launch_fused(const FusedArgs& a, int grid_ndim, int src_ndim, int mov
                         hipStream_t st) {
     if (a.tiles == 0 || n_poses <= 0) return hipSuccess;
     if (grid_ndim == 3 && (a.zchunk <= 0 || a.zchunk % kZBlock != 0)) return hipErrorInvalidValue;
+    if (grid_ndim == 2 && a.zchunk <= 0) return hipErrorInvalidValue;
     FusedArgs b = a;
     b.npose = n_poses;
     const long long tiles8 = (a.tiles + kXcds - 1) / kXcds * (long long)kXcds;
