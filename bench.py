@@ -203,7 +203,7 @@ def _serve_poisson(S, eng, classes, lam, max_poses, slots, duration_s, seed):
         srv.wait(ticket)
         lat[c].append(time.perf_counter() - t_arr)
 
-    with ThreadPoolExecutor(max_workers=32) as pool:
+    with ThreadPoolExecutor(max_workers=256) as pool:   # one waiter per outstanding request
         t0 = time.perf_counter()
         for a, c in zip(arrivals, kind):
             now = time.perf_counter() - t0
@@ -234,9 +234,10 @@ def serving_figures(eng, E, synth, pair3d, mats3d, w_ct):
           MR 256x256x192 stored f16;
       20% 3D: US 192x192x160 u8 against MR 256x256x192 f32, or US 256^3 u8 against the
           int16 CT 512x512x400 (stored f16).
-    Open-loop Poisson arrivals at 5, 20, 50, 100 and 200 requests/s through continuous
+    Open-loop Poisson arrivals at 5, 20, 50, 100, 200 and 1000 requests/s through continuous
     batching (each pair's queued requests merged into one launch of up to 64 poses, up to
-    4 launches of different pairs on the device at once), and at 200 requests/s through a
+    4 launches of different pairs on the device at once while their work-groups fit it
+    together), and at 200 and 1000 requests/s through a
     plain queue (one request per launch, one launch at a time); then 8 closed-loop clients
     on the 3D MR pair, in both modes."""
     from lc2registration_amd import serve as S
@@ -282,9 +283,11 @@ def serving_figures(eng, E, synth, pair3d, mats3d, w_ct):
         for k in range(1, 9):
             p.eval_batch(mm[:k])
     out = {"serve_resident_bytes": used, "serve_poisson": {}}
-    for lam, mp, sl in ((5, 64, 4), (20, 64, 4), (50, 64, 4), (100, 64, 4), (200, 64, 4), (200, 1, 1)):
+    for lam, mp, sl in ((5, 64, 4), (20, 64, 4), (50, 64, 4), (100, 64, 4), (200, 64, 4),
+                        (200, 1, 1), (1000, 64, 4), (1000, 1, 1)):
         key = f"{lam}_per_s" + ("_continuous" if mp > 1 else "_queue_one_launch_at_a_time")
-        r = _serve_poisson(S, eng, classes, lam, mp, sl, max(2.0, 60.0 / lam), seed=lam + mp)
+        # the same arrivals and requests for both modes at one rate
+        r = _serve_poisson(S, eng, classes, lam, mp, sl, max(2.0, 60.0 / lam), seed=lam)
         out["serve_poisson"][key] = r
         log(f"serving Poisson {key}: " + ", ".join(
             f"{n} {r[n]['p50_ms']:.2f}/{r[n]['p99_ms']:.2f}" for n, *_ in classes

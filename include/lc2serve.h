@@ -1,8 +1,10 @@
 /* Serving loop over the LC2 engine (lc2e.h): many client threads submit candidate pose
  * sets for resident pairs. One worker thread merges the queued requests of a pair into a
  * single fused launch and keeps launches of different pairs in flight on the device at
- * once, each on its own stream (a launch slot). Requests that arrive while others are on
- * the GPU join the next launch of their pair as a slot frees up (continuous batching).
+ * once, each on its own stream (a launch slot), while their work-groups fit the device
+ * together; a launch that fills the GPU by itself runs alone. Requests that arrive while
+ * others are on the GPU join the next launch of their pair as a slot and room on the
+ * device free up (continuous batching).
  *
  * No counterpart in the reference: it evaluates one pose at a time from one optimizer
  * (optimizers.cu:130-177, :650-728) and has no request path. Its closest component, the

@@ -132,6 +132,7 @@ struct lc2e_pair {
     int* mi_ext = nullptr;
     unsigned* mi_hist = nullptr;
     double* mi_out = nullptr;
+    int resident_per_cu = 0;  // fused work-groups one CU holds at once (0: not queried yet)
     // kernel timing (eager path)
     bool profile = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -372,6 +373,24 @@ void set_last_error(const std::string& msg) { g_last_error = msg; }
 int engine_alloc(lc2e_engine* e, void** p, size_t bytes) { return e->alloc(p, bytes); }
 void engine_release(lc2e_engine* e, void* p, size_t bytes) { e->release(p, bytes); }
 int engine_device(lc2e_engine* e) { return e->device; }
+// The share of the device one fused launch of n poses occupies: its work-groups over the
+// work-groups all CUs hold at once (>= 1: it fills the GPU by itself). The serving loop
+// runs launches side by side only while their shares add up to at most 1.
+double launch_share(lc2e_pair* p, int n) {
+    const Tiling t = tiling_for(p, n);
+    if (t.tiles <= 0 || n <= 0) return 0.0;
+    if (p->resident_per_cu <= 0) {
+        const FusedArgs a = make_args(p, nullptr, nullptr, t);
+        int per_cu = 0;
+        if (fused_resident_per_cu(a, p->info.grid_ndim, p->info.src_ndim,
+                                  p->info.high_res_is_moving, &per_cu) != hipSuccess || per_cu <= 0)
+            per_cu = 1;
+        p->resident_per_cu = per_cu;
+    }
+    const double wgs = (double)((t.tiles + kXcds - 1) / kXcds * kXcds) * n;
+    const int cus = p->e->cus > 0 ? p->e->cus : 256;
+    return wgs / ((double)cus * p->resident_per_cu);
+}
 }  // namespace lc2e
 
 extern "C" {

@@ -1409,26 +1409,35 @@ constexpr size_t lds_bytes_2d(int r, bool movi) {
 }
 static_assert(lds_bytes_2d(kMaxR2, false) <= 160 * 1024, "2D fused kernel exceeds the 160 KiB of LDS");
 
+// The launch helpers below either launch (occ == nullptr) or report, in *occ, how many
+// work-groups of the kernel they would launch fit on one CU at once.
 template <int R, typename T>
-hipError_t launch3(const FusedArgs& a, bool movi, dim3 grid, hipStream_t st) {
-    if (movi) hipLaunchKernelGGL((k_fused3d<R, true, T, kTile3X, kTile3Y>), grid, dim3(kTile3X * kTile3Y), 0, st, a);
-    else hipLaunchKernelGGL((k_fused3d<R, false, T, kTile3X, kTile3Y>), grid, dim3(kTile3X * kTile3Y), 0, st, a);
+hipError_t launch3(const FusedArgs& a, bool movi, dim3 grid, hipStream_t st, int* occ) {
+    constexpr int nt = kTile3X * kTile3Y;
+    if (occ) {
+        return movi ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          occ, k_fused3d<R, true, T, kTile3X, kTile3Y>, nt, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          occ, k_fused3d<R, false, T, kTile3X, kTile3Y>, nt, 0);
+    }
+    if (movi) hipLaunchKernelGGL((k_fused3d<R, true, T, kTile3X, kTile3Y>), grid, dim3(nt), 0, st, a);
+    else hipLaunchKernelGGL((k_fused3d<R, false, T, kTile3X, kTile3Y>), grid, dim3(nt), 0, st, a);
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch3r(const FusedArgs& a, bool movi, dim3 grid, hipStream_t st) {
+hipError_t launch3r(const FusedArgs& a, bool movi, dim3 grid, hipStream_t st, int* occ) {
     switch (a.r) {
-        case 1: return launch3<1, T>(a, movi, grid, st);
-        case 2: return launch3<2, T>(a, movi, grid, st);
-        case 3: return launch3<3, T>(a, movi, grid, st);
-        case 4: return launch3<4, T>(a, movi, grid, st);
+        case 1: return launch3<1, T>(a, movi, grid, st, occ);
+        case 2: return launch3<2, T>(a, movi, grid, st, occ);
+        case 3: return launch3<3, T>(a, movi, grid, st, occ);
+        case 4: return launch3<4, T>(a, movi, grid, st, occ);
         default: return hipErrorInvalidValue;
     }
 }
 
 template <bool MOVI, bool SRC3D, typename T, int RMAX>
-hipError_t launch2r(const FusedArgs& a, dim3 grid, hipStream_t st) {
+hipError_t launch2r(const FusedArgs& a, dim3 grid, hipStream_t st, int* occ) {
     const size_t lds = lds_bytes_2d(a.r, MOVI);
     static bool configured = false;  // allow this variant's largest radius once
     if (!configured) {
@@ -1438,22 +1447,25 @@ hipError_t launch2r(const FusedArgs& a, dim3 grid, hipStream_t st) {
         if (e != hipSuccess) return e;
         configured = true;
     }
+    if (occ) return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_fused2d<MOVI, SRC3D, T, RMAX>,
+                                                                  kThreads, lds);
     hipLaunchKernelGGL((k_fused2d<MOVI, SRC3D, T, RMAX>), grid, dim3(kThreads), lds, st, a);
     return hipGetLastError();
 }
 
 // Radius classes of the 2D kernel: fewer registers for the common small patches.
 template <bool MOVI, bool SRC3D, typename T>
-hipError_t launch2(const FusedArgs& a, dim3 grid, hipStream_t st) {
-    if (a.r <= 3) return launch2r<MOVI, SRC3D, T, 3>(a, grid, st);
-    if (a.r <= 6) return launch2r<MOVI, SRC3D, T, 6>(a, grid, st);
-    return launch2r<MOVI, SRC3D, T, kMaxR2>(a, grid, st);
+hipError_t launch2(const FusedArgs& a, dim3 grid, hipStream_t st, int* occ) {
+    if (a.r <= 3) return launch2r<MOVI, SRC3D, T, 3>(a, grid, st, occ);
+    if (a.r <= 6) return launch2r<MOVI, SRC3D, T, 6>(a, grid, st, occ);
+    return launch2r<MOVI, SRC3D, T, kMaxR2>(a, grid, st, occ);
 }
 
 template <typename T>
-hipError_t launch2t(const FusedArgs& a, bool movi, int src_ndim, dim3 grid, hipStream_t st) {
-    if (movi) return src_ndim == 3 ? launch2<true, true, T>(a, grid, st) : launch2<true, false, T>(a, grid, st);
-    return src_ndim == 3 ? launch2<false, true, T>(a, grid, st) : launch2<false, false, T>(a, grid, st);
+hipError_t launch2t(const FusedArgs& a, bool movi, int src_ndim, dim3 grid, hipStream_t st, int* occ) {
+    if (movi)
+        return src_ndim == 3 ? launch2<true, true, T>(a, grid, st, occ) : launch2<true, false, T>(a, grid, st, occ);
+    return src_ndim == 3 ? launch2<false, true, T>(a, grid, st, occ) : launch2<false, false, T>(a, grid, st, occ);
 }
 
 }  // namespace
@@ -1492,10 +1504,22 @@ hipError_t launch_fused(const FusedArgs& a, int grid_ndim, int src_ndim, int mov
     if (tiles8 * n_poses > 0x7fffffffLL) return hipErrorInvalidValue;
     const dim3 grid((unsigned)(tiles8 * n_poses));
     if (grid_ndim == 3)
-        return b.src_f16 ? launch3r<__half>(b, movi, grid, st) : launch3r<float>(b, movi, grid, st);
+        return b.src_f16 ? launch3r<__half>(b, movi, grid, st, nullptr)
+                         : launch3r<float>(b, movi, grid, st, nullptr);
     if (b.r < 1 || b.r > kMaxR2) return hipErrorInvalidValue;
-    return b.src_f16 ? launch2t<__half>(b, movi, src_ndim, grid, st)
-                     : launch2t<float>(b, movi, src_ndim, grid, st);
+    return b.src_f16 ? launch2t<__half>(b, movi, src_ndim, grid, st, nullptr)
+                     : launch2t<float>(b, movi, src_ndim, grid, st, nullptr);
+}
+
+hipError_t fused_resident_per_cu(const FusedArgs& a, int grid_ndim, int src_ndim, int movi,
+                                 int* per_cu) {
+    *per_cu = 0;
+    if (grid_ndim == 3)
+        return a.src_f16 ? launch3r<__half>(a, movi, dim3(1), nullptr, per_cu)
+                         : launch3r<float>(a, movi, dim3(1), nullptr, per_cu);
+    if (a.r < 1 || a.r > kMaxR2) return hipErrorInvalidValue;
+    return a.src_f16 ? launch2t<__half>(a, movi, src_ndim, dim3(1), nullptr, per_cu)
+                     : launch2t<float>(a, movi, src_ndim, dim3(1), nullptr, per_cu);
 }
 
 hipError_t launch_reduce(const double* partials, int blocks_per_pose, int n_poses, double* out,

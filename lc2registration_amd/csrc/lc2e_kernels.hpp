@@ -75,6 +75,10 @@ int fused_blocks_2d(const VGeom& v, int r, int* ntx, int* nty);
 
 hipError_t launch_fused(const FusedArgs& a, int grid_ndim, int src_ndim, int high_res_is_moving,
                         int n_poses, hipStream_t st);
+// Work-groups of the fused kernel a launch with these arguments would use that fit on one
+// CU at once (hipOccupancyMaxActiveBlocksPerMultiprocessor).
+hipError_t fused_resident_per_cu(const FusedArgs& a, int grid_ndim, int src_ndim, int movi,
+                                 int* per_cu);
 hipError_t launch_reduce(const double* partials, int blocks_per_pose, int n_poses, double* out,
                          hipStream_t st);
 // Fixed image -> virtual grid at identity (frames of a 2D stack one by one).

@@ -1,8 +1,9 @@
 // Serving loop (include/lc2serve.h). One worker thread owns a few launch slots, each with
 // its own HIP stream and device/pinned staging buffers. It merges the queued requests of a
 // pair into one lc2e_eval_batch_async launch on a free slot, keeps up to `slots` launches
-// of different pairs in flight on the device at once, and completes requests as their
-// launches finish. Requests that arrive meanwhile are merged into the next launch of
+// of different pairs in flight on the device at once while their work-groups fit the
+// device together (a launch that fills the GPU runs alone), and completes requests as
+// their launches finish. Requests that arrive meanwhile are merged into the next launch of
 // their pair (continuous batching at launch granularity).
 #include <hip/hip_runtime.h>
 
@@ -28,6 +29,7 @@ void set_last_error(const std::string& msg);
 int engine_alloc(lc2e_engine* e, void** p, size_t bytes);
 void engine_release(lc2e_engine* e, void* p, size_t bytes);
 int engine_device(lc2e_engine* e);
+double launch_share(lc2e_pair* p, int n);
 }  // namespace lc2e
 
 namespace {
@@ -67,6 +69,7 @@ struct Slot {
     lc2e_pair* pair = nullptr;
     std::vector<RequestPtr> batch;
     int total = 0;
+    double share = 0.0;  // of the device (lc2e::launch_share)
 };
 
 void fill_error(Request& r, int rc, const std::string& err) {
@@ -187,6 +190,18 @@ struct lc2s_server {
         return nullptr;
     }
 
+    // Under the lock: the poses take() would put into one launch of the pair.
+    int peek_total(lc2e_pair* pair) const {
+        int total = 0;
+        bool first = true;
+        for (const auto& r : queue) {
+            if (r->pair != pair) continue;
+            if (first || total + r->n <= max_poses) total += r->n;
+            first = false;
+        }
+        return total;
+    }
+
     // Under the lock: move the pair's queued requests, in order, into the slot while they
     // fit one launch (a request larger than max_poses goes alone).
     void take(Slot& s, lc2e_pair* pair) {
@@ -276,15 +291,23 @@ struct lc2s_server {
         (void)hipSetDevice(device);
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
-            // 1. fill free slots with the oldest dispatchable pairs
+            // 1. fill free slots with the oldest dispatchable pairs, while the launches in
+            // flight and the next one together fit the device (a launch that fills it alone
+            // runs alone); the oldest request that does not fit waits, so none is passed over
             std::vector<Slot*> started;
             Clock::time_point wake{};
             bool held = false;
+            double busy_share = 0.0;
+            for (Slot& s : slots) busy_share += s.busy ? s.share : 0.0;
             for (Slot& s : slots) {
                 if (s.busy) continue;
                 lc2e_pair* pair = pick(Clock::now(), wake, held);
                 if (!pair) break;
+                const double share = lc2e::launch_share(pair, peek_total(pair));
+                if (busy_share > 0.0 && busy_share + share > 1.0) break;
                 take(s, pair);
+                s.share = share;
+                busy_share += share;
                 in_flight.insert(pair);
                 started.push_back(&s);
             }
