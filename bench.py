@@ -370,12 +370,20 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds per CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip single-pose / fp16 figures")
+    ap.add_argument("--sweep-only", action="store_true",
+                    help="only the tracked-sweep figure (2D kernel), timed over --steps launches "
+                         "(for profiling that kernel)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES three times (the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.sweep_only:
+        from lc2registration_amd import engine as E
+        from lc2registration_amd import synth
+        print(json.dumps(sweep_figures(E.Engine(0), E, synth, steps=args.steps)), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1",

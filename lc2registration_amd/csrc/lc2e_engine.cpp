@@ -188,20 +188,6 @@ int model_zchunk(const lc2e_pair* p, int n) {
     return best;
 }
 
-// Frames per 2D work-group (frame stacks): the 2D kernel pipelines the gathers of the
-// next frame of its run behind the current one, so runs are long (16 frames) unless that
-// leaves fewer than 8 work-groups per CU.
-int model_frame_run(const lc2e_pair* p, int n) {
-    int ntx, nty;
-    const int F = p->v.Z;
-    if (F <= 1 || lc2e::fused_blocks_2d(p->v, p->info.patch_radius, &ntx, &nty) <= 0) return 1;
-    const long long cols = (long long)ntx * nty;
-    const int cus = p->e->cus > 0 ? p->e->cus : 256;
-    int run = 16;
-    while (run > 1 && cols * ((F + run - 1) / run) * n < 8LL * cus) run /= 2;
-    return run < F ? run : F;
-}
-
 Tiling tiling_for(lc2e_pair* p, int n) {
     Tiling t;
     const int r = p->info.patch_radius;
@@ -213,11 +199,12 @@ Tiling tiling_for(lc2e_pair* p, int n) {
         t.bpp = lc2e::fused_partials_3d(p->v, r);
         p->last_zchunk = t.zchunk;
     } else {
-        // partial sums per frame and tile; work-groups per run of frames and tile
-        t.bpp = lc2e::fused_blocks_2d(p->v, r, &t.ntx, &t.nty);
-        t.zchunk = model_frame_run(p, n);
-        t.nzc = (p->v.Z + t.zchunk - 1) / t.zchunk;
-        t.tiles = t.bpp > 0 ? t.ntx * t.nty * t.nzc : 0;
+        // one work-group and one partial per tile and run of kFrameRun frames (fixed, so
+        // that the summation order does not depend on the batch)
+        const int tf = lc2e::fused_blocks_2d(p->v, r, &t.ntx, &t.nty);
+        t.zchunk = p->v.Z < lc2e::kFrameRun ? p->v.Z : lc2e::kFrameRun;
+        t.nzc = tf > 0 ? (p->v.Z + t.zchunk - 1) / t.zchunk : 0;
+        t.tiles = t.bpp = t.ntx * t.nty * t.nzc;
     }
     return t;
 }

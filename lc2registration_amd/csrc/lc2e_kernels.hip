@@ -594,17 +594,19 @@ __device__ __forceinline__ void box_xy(const double* s_prod, const int* s_cnt, d
     __syncthreads();
 }
 
-// box_xy for a patch radius known only at run time (the 2D kernel), 1 <= r <= kMaxR2.
+// box_xy for a patch radius known only at run time (the 2D kernel), 1 <= r <= RMAX, over
+// the (TX + 2r) x (TY + 2r) products region of row stride odd_stride(TX + 2r): the strides
+// become compile-time constants of each radius.
 template <int NC, bool CNT, int TX, int TY, int NTHR, int RMAX = kMaxR2, int RR = 1>
 __device__ __forceinline__ void box_xy_r(int r, const double* s_prod, const int* s_cnt,
                                          double* s_xp, int* s_xcnt, double* s_out, int* s_ocnt,
-                                         int ps, int py, int tid) {
+                                         int tid) {
     if (r == RR)
-        box_xy<NC, CNT, TX, TY, NTHR, 2 * RR + 1>(s_prod, s_cnt, s_xp, s_xcnt, s_out, s_ocnt, ps,
-                                                  py, tid);
-    else if constexpr (RR < kMaxR2)
-        box_xy_r<NC, CNT, TX, TY, NTHR, RMAX, RR + 1>(r, s_prod, s_cnt, s_xp, s_xcnt, s_out, s_ocnt, ps,
-                                                py, tid);
+        box_xy<NC, CNT, TX, TY, NTHR, 2 * RR + 1>(s_prod, s_cnt, s_xp, s_xcnt, s_out, s_ocnt,
+                                                  odd_stride(TX + 2 * RR), TY + 2 * RR, tid);
+    else if constexpr (RR < RMAX)
+        box_xy_r<NC, CNT, TX, TY, NTHR, RMAX, RR + 1>(r, s_prod, s_cnt, s_xp, s_xcnt, s_out, s_ocnt,
+                                                      tid);
 }
 
 // ---------------------------------------------------------------------------
@@ -864,12 +866,13 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// 2D fused kernel: one 16x16 output tile of a run of `zchunk` consecutive frames of one
-// pose (a single image is one frame). Dynamic LDS sized for the runtime patch radius.
-// Frame f + 1's gathers (and its static image values and ultrasound statistics) are
-// issued while frame f's box sums and combine run, as in the 3D kernel's z-steps. Each
-// frame's tile sum goes to its own partial (frame * tiles + tile), so results do not
-// depend on the run length.
+// 2D fused kernel: one 16x16 output tile of a run of `zchunk` (kFrameRun) consecutive
+// frames of one pose (a single image is one frame). Dynamic LDS sized for the runtime patch
+// radius. Frame f + 1's gathers (and its static image values and ultrasound statistics)
+// are issued while frame f's box sums and combine run, as in the 3D kernel's z-steps. Each
+// thread adds its pixel's measure and weight over the run's frames in frame order, and the
+// work-group reduces them once into the partial of (run, tile). The run length does not
+// depend on the batch, so neither do the results.
 // ---------------------------------------------------------------------------
 template <bool MOVI, bool SRC3D, typename T, int RMAX>
 __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
@@ -910,6 +913,33 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
     const T* __restrict__ src = static_cast<const T*>(a.src);
     const float* mats = a.mats + (size_t)pose * a.mats_per_pose * 12;
 
+    // Per-thread positions, the same for every frame (the radius is a run-time value, so
+    // the divisions are done once here, not per frame): virtual-grid coordinates of the
+    // warped positions, and the LDS / image offsets of the products positions.
+    float gLx[SPT], gLy[SPT];
+    unsigned g_in = 0;  // bit j: warped position j lies in the grid
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+        const int i = tid + j * kThreads;
+        const int lx = i % wex, ly = i / wex;
+        const int gx = x0 - wofs + lx, gy = y0 - wofs + ly;
+        if (i < NI && gx >= 0 && gx < X && gy >= 0 && gy < Y) g_in |= 1u << j;
+        gLx[j] = fmaf((float)gx, a.v.sx, a.v.ox);
+        gLy[j] = fmaf((float)gy, a.v.sy, a.v.oy);
+    }
+    int p_in_ofs[PPT], p_prod_ofs[PPT], p_img_ofs[PPT];  // s_in, s_prod, image (frame 0)
+    unsigned p_in = 0;  // bit j: products position j holds an image value
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        const int i = tid + j * kThreads;
+        const int lx = i % PX, ly = i / PX;
+        const int gx = x0 - R + lx, gy = y0 - R + ly;
+        if (i < NP && gx < X && gy < Y) p_in |= 1u << j;
+        p_in_ofs[j] = MOVI ? ly * IXE + lx : i;
+        p_prod_ofs[j] = ly * PS + lx;
+        p_img_ofs[j] = gy * X + gx;
+    }
+
     using Pend = typename std::conditional<SRC3D, Pending3<T>, Pending2<T>>::type;
     Pend pend[SPT];
     auto issue_frame = [&](int frame) {
@@ -920,13 +950,9 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
 #pragma unroll
         for (int j = 0; j < SPT; ++j) {
             if (j * kThreads >= NI) break;  // uniform
-            const int i = tid + j * kThreads;
-            const int lx = i % wex, ly = i / wex;
-            const int gx = x0 - wofs + lx, gy = y0 - wofs + ly;
-            const bool inside = i < NI && gx >= 0 && gx < X && gy >= 0 && gy < Y;
-            const float Lx = fmaf((float)gx, a.v.sx, a.v.ox), Ly = fmaf((float)gy, a.v.sy, a.v.oy);
-            if constexpr (SRC3D) issue3(src, a.s, m, Lx, Ly, lz, inside, pend[j]);
-            else issue2(src, a.s, m, Lx, Ly, inside, pend[j]);
+            const bool inside = (g_in >> j) & 1u;
+            if constexpr (SRC3D) issue3(src, a.s, m, gLx[j], gLy[j], lz, inside, pend[j]);
+            else issue2(src, a.s, m, gLx[j], gLy[j], inside, pend[j]);
         }
     };
     // static image values of the products positions (U if MOVI, else I and |grad I|)
@@ -936,11 +962,8 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
             if (j * kThreads >= NP) break;  // uniform
-            const int i = tid + j * kThreads;
-            const int lx = i % PX, ly = i / PX;
-            const int gx = x0 - R + lx, gy = y0 - R + ly;
-            const bool in = i < NP && gx < X && gy < Y;
-            const int vidx = plane + gy * X + gx;
+            const bool in = (p_in >> j) & 1u;
+            const int vidx = plane + p_img_ofs[j];
             pre0[j] = in ? a.stat_img[vidx] : 0.0f;
             if (!MOVI) pre1[j] = in ? a.stat_grad[vidx] : 0.0f;
         }
@@ -957,6 +980,7 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
         }
     };
 
+    double acc_m = 0.0, acc_w = 0.0;
     issue_frame(f0);
     issue_static(f0);
     issue_stats(f0);
@@ -989,14 +1013,13 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
             if (j * kThreads >= NP) break;  // uniform
             const int i = tid + j * kThreads;
             if (i < NP) {
-                const int lx = i % PX, ly = i / PX;
                 float I, G, U;
                 if (MOVI) {
                     // 3x3 Sobel read from the warped tile: rows smooth / difference along x
                     float sv[3], dv[3], cc = 0.0f;
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        const float* row = s_in + (ly + k) * IXE + lx;
+                        const float* row = s_in + p_in_ofs[j] + k * IXE;
                         const float l = row[0], c = row[1], r = row[2];
                         dv[k] = r - l;
                         sv[k] = (l + 2.0f * c) + r;
@@ -1008,11 +1031,11 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
                     I = cc;
                     U = cur0[j];
                 } else {
-                    U = s_in[i];
+                    U = s_in[p_in_ofs[j]];
                     I = cur0[j];
                     G = cur1[j];
                 }
-                store_products<NC>(s_prod, s_cnt, PX * PS, ly * PS + lx, I, G, U);
+                store_products<NC>(s_prod, s_cnt, PX * PS, p_prod_ofs[j], I, G, U);
             }
         }
         __syncthreads();  // s_in reads done; products visible
@@ -1020,9 +1043,8 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
         if (f + 1 < f1) issue_frame(f + 1);
         // 3. box sums
         box_xy_r<NC, CNT, kTile, kTile, kThreads, RMAX>(R, s_prod, s_cnt, s_xp, s_xcnt, s_prod,
-                                                        s_cnt, PS, PX, tid);
-        // 4. combine, and the frame's tile sum
-        double acc_m = 0.0, acc_w = 0.0;
+                                                        s_cnt, tid);
+        // 4. combine; each thread adds its pixel's values over the run's frames in order
         if (out_ok) {
             double S[NC];
 #pragma unroll
@@ -1032,8 +1054,8 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
             if (MOVI) ok = combine_static_u(S, mu, cyy, W * W, w, meas);
             else ok = combine(S, s_cnt[tid], W * W, w, meas);
             if (ok) {
-                acc_m = meas;
-                acc_w = w;
+                acc_m += meas;
+                acc_w += w;
                 if (a.wmap) {
                     const int vo = f * X * Y + (y0 + oy) * X + (x0 + ox);
                     a.wmap[vo] = w;
@@ -1041,9 +1063,9 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
                 }
             }
         }
-        double* dst = a.partials + ((size_t)pose * a.blocks_per_pose + (size_t)f * cols + tcol) * 2;
-        block_reduce_store<kThreads>(acc_m, acc_w, s_red, dst);
     }
+    double* dst = a.partials + ((size_t)pose * a.blocks_per_pose + (size_t)run * cols + tcol) * 2;
+    block_reduce_store<kThreads>(acc_m, acc_w, s_red, dst);
 }
 
 // Per-pose fixed-order reduction of the work-group partials -> (sum m, sum w, LC2)

@@ -18,6 +18,7 @@ constexpr int kMaxR3 = 4;      // largest patch radius of the 3D fused kernel
 constexpr int kZBlock = 8;     // 3D partial sums per block of 8 output planes; z-chunks are multiples
 constexpr int kXcds = 8;       // MI355X: 8 XCDs, each with its own L2, 32 CUs each
 constexpr int kMaxR2 = 10;     // largest patch radius of the 2D fused kernel
+constexpr int kFrameRun = 16;  // frames per 2D work-group (frame stacks)
 
 struct VGeom {  // virtual grid
     int X, Y, Z;
