@@ -343,6 +343,52 @@ def ct_precision_figures(eng, E, synth, w, poses=32, steps=3):
     return out
 
 
+def registration_3d_figures(w, n_runs=3, seed=41):
+    """Whole 3D registrations through the product path (lc2reg_register_yaml) on the 3D
+    affine workload (MR 256x256x192 f32 against US 192x192x160 u8, 7^3 patch): from the
+    generating pose moved by up to 3 mm / 3 deg in each rigid parameter, with identity
+    scale and shear, two BOBYQA-stage rounds (rigid, then all 12 parameters). Returns
+    p50 / p99 wall time, evaluations, and the largest error against the generating pose."""
+    import tempfile
+    from lc2registration_amd import registration as REG
+    rng = np.random.default_rng(seed)
+    f, m = w.fixed, w.moving
+    truth = np.asarray(w.true_pose, np.float64)
+    times, evals, errs = [], [], []
+    with tempfile.TemporaryDirectory() as tmp:
+        fp, mp = os.path.join(tmp, "us.raw"), os.path.join(tmp, "mr.raw")
+        f.data.tofile(fp)
+        m.data.astype(np.float32).tofile(mp)
+        for k in range(n_runs):
+            init = np.r_[truth[:6] + rng.uniform(-3, 3, 6), 1, 1, 1, 0, 0, 0]
+            cfg = REG.write_parameters_yaml(
+                os.path.join(tmp, f"p{k}.yaml"), 3, (fp, "uint8", f.dims, f.spacing, f.origin),
+                (mp, "float32", m.dims, m.spacing, m.origin), "Moving", w.resampled_spacing,
+                w.patch_radius, tmp, False, init, [-10] * 6 + [0.9] * 3 + [-0.1] * 3,
+                [10] * 6 + [1.1] * 3 + [0.1] * 3,
+                rounds=[((True, True, False, False), 1e-4), ((True, True, True, True), 1e-4)])
+            t = time.perf_counter()
+            x, lc2, nev, _ = REG.register_yaml("3D", cfg, write_images=False)
+            times.append(time.perf_counter() - t)
+            evals.append(nev)
+            errs.append(np.abs(x - truth))
+    ms = np.array(times) * 1e3
+    e = np.array(errs)
+    out = {"registration_3d_p50_ms": float(np.percentile(ms, 50)),
+           "registration_3d_p99_ms": float(np.percentile(ms, 99)),
+           "registration_3d_evaluations": [int(v) for v in evals],
+           "registration_3d_max_error_mm": float(e[:, :3].max()),
+           "registration_3d_max_error_deg": float(e[:, 3:6].max()),
+           "registration_3d_max_error_scale_shear": float(e[:, 6:].max()),
+           "registration_3d_config": "3D affine workload, from the generating pose moved by "
+                                     "<= 3 mm / 3 deg per rigid parameter, BOBYQA-stage rounds "
+                                     f"(rigid, all 12), {n_runs} runs"}
+    log(f"3D registration latency p50 {out['registration_3d_p50_ms']:.0f} ms, "
+        f"{evals} evaluations, max error {out['registration_3d_max_error_mm']:.3f} mm / "
+        f"{out['registration_3d_max_error_deg']:.3f} deg")
+    return out
+
+
 def calibrate_copy(nbytes):
     """A copy kernel of known size: under rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per
     dispatch, divided by nbytes, give the factor that turns those counters into bytes."""
@@ -482,6 +528,7 @@ def main():
         del w_ct
         extra.update(sweep_figures(eng, E, synth))
         extra.update(registration_latency_figures(synth))
+        extra.update(registration_3d_figures(w))
 
     cpu = None
     if not args.no_cpu_baseline:
