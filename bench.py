@@ -29,6 +29,7 @@ ALG_BYTES_PER_VOXEL = 1.0 + 0.5
 ALG_F32_PER_VOXEL = 116.0   # warp 40, Sobel 27, products 5, window sums 42, reduction 2
 ALG_F64_PER_VOXEL = 45.0    # centred moments, 2x2 solve, variance/measure, thresholds
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "k_fused3d_r3_affine3d_us192_traffic.json")
+SIMDS, CLOCK_HZ = 256 * 4, 2.4e9  # MI355X: 256 CUs of 4 SIMDs, peak engine clock
 
 
 def log(msg):
@@ -483,12 +484,13 @@ def main():
     alg_bytes = P * V * ALG_BYTES_PER_VOXEL
     achieved = alg_bytes / avg_kernel_s / 1e9
     t_valu = P * V * (ALG_F32_PER_VOXEL / FP32_PEAK + ALG_F64_PER_VOXEL / FP64_PEAK)
-    traffic = None
+    traffic, valu_insts = None, None
     if os.path.exists(PROFILE_TRAFFIC):
         with open(PROFILE_TRAFFIC) as fh:
             tr = json.load(fh)
         if tr.get("poses") == P:
             traffic = tr.get("hbm_bytes_per_launch")
+            valu_insts = tr.get("sq_insts_valu_per_launch")
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "kernel": "k_fused3d<R=3, moving high-res>",
@@ -496,7 +498,12 @@ def main():
                 "algorithmic_bytes_per_launch": alg_bytes,
                 "valu_frac": t_valu / avg_kernel_s,
                 "valu_note": "algorithmic FP32 116 + FP64 45 flop per voxel-pose vs "
-                             "157.3 / 78.6 TFLOP/s vector peaks"}
+                             "157.3 / 78.6 TFLOP/s vector peaks",
+                # the binding resource: VALU issue. SQ_INSTS_VALU of the profiled launch
+                # (profiles/), 4 cycles per wave64 instruction on a 16-lane SIMD, over
+                # 1024 SIMDs at 2.4 GHz
+                "valu_issue_frac": (valu_insts * 4 / (SIMDS * CLOCK_HZ) / avg_kernel_s
+                                    if valu_insts else None)}
 
     extra = {}
     if not args.no_extra:
