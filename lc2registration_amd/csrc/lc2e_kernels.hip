@@ -210,11 +210,22 @@ __device__ __forceinline__ void ld2_off(const T* __restrict__ base, unsigned idx
 }
 
 template <typename T>
+__device__ __forceinline__ void issue3v(const T* __restrict__ src, const SGeom& s, float px,
+                                        float py, float pz, bool inside, Pending3<T>& p);
+
+template <typename T>
 __device__ __forceinline__ void issue3(const T* __restrict__ src, const SGeom& s, const float* m,
                                        float lx, float ly, float lz, bool inside, Pending3<T>& p) {
     const float px = fmaf(m[2], lz, fmaf(m[1], ly, m[0] * lx)) + m[3];
     const float py = fmaf(m[6], lz, fmaf(m[5], ly, m[4] * lx)) + m[7];
     const float pz = fmaf(m[10], lz, fmaf(m[9], ly, m[8] * lx)) + m[11];
+    issue3v(src, s, px, py, pz, inside, p);
+}
+
+// issue3 from the source position (px, py, pz) in mm.
+template <typename T>
+__device__ __forceinline__ void issue3v(const T* __restrict__ src, const SGeom& s, float px,
+                                        float py, float pz, bool inside, Pending3<T>& p) {
     float vx, vy, vz;
     to_voxel(s, px, py, pz, vx, vy, vz);
     const float xm = (float)(s.X - 1), ym = (float)(s.Y - 1), zm = (float)(s.Z - 1);
@@ -245,6 +256,17 @@ __device__ __forceinline__ void issue3(const T* __restrict__ src, const SGeom& s
         ld2_off(src, ok ? r10 : tail, p.c0, p.c1);
         ld2_off(src, ok ? r11 : tail, p.d0, p.d1);
     }
+}
+
+// issue3 with the in-plane part of the source coordinates given: c = m*(lx, ly, 0) as
+// issue3 forms it, so px = fmaf(m[2], lz, c[0]) + m[3] is issue3's px bit for bit.
+template <typename T>
+__device__ __forceinline__ void issue3c(const T* __restrict__ src, const SGeom& s, const float* m,
+                                        const float* c, float lz, bool inside, Pending3<T>& p) {
+    const float px = fmaf(m[2], lz, c[0]) + m[3];
+    const float py = fmaf(m[6], lz, c[1]) + m[7];
+    const float pz = fmaf(m[10], lz, c[2]) + m[11];
+    issue3v(src, s, px, py, pz, inside, p);
 }
 
 template <typename T>
@@ -666,18 +688,27 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
     const int nsteps = nout + 2 * R + LAG;
 
     Pending3<T> pend[SPT];
+    // the in-plane part of each warped position's source coordinates, m*(Lx, Ly, 0), is the
+    // same for every plane: computed once, in the order issue3 uses
+    float cxy[SPT][3];
+    unsigned in_xy = 0;
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+        const int i = tid + j * NTHR;
+        const int lx = i % wex, ly = i / wex;
+        const int gx = x0 - wofs + lx, gy = y0 - wofs + ly;
+        if (i < NI && gx >= 0 && gx < X && gy >= 0 && gy < Y) in_xy |= 1u << j;
+        const float Lx = fmaf((float)gx, a.v.sx, a.v.ox), Ly = fmaf((float)gy, a.v.sy, a.v.oy);
+        cxy[j][0] = fmaf(m[1], Ly, m[0] * Lx);
+        cxy[j][1] = fmaf(m[5], Ly, m[4] * Lx);
+        cxy[j][2] = fmaf(m[9], Ly, m[8] * Lx);
+    }
     auto issue_plane = [&](int zi) {
         const bool zin = zi >= 0 && zi < Z;
         const float lz = fmaf((float)zi, a.v.sz, a.v.oz);
 #pragma unroll
-        for (int j = 0; j < SPT; ++j) {
-            const int i = tid + j * NTHR;
-            const int lx = i % wex, ly = i / wex;
-            const int gx = x0 - wofs + lx, gy = y0 - wofs + ly;
-            const bool inside = i < NI && zin && gx >= 0 && gx < X && gy >= 0 && gy < Y;
-            issue3(src, a.s, m, fmaf((float)gx, a.v.sx, a.v.ox), fmaf((float)gy, a.v.sy, a.v.oy),
-                   lz, inside, pend[j]);
-        }
+        for (int j = 0; j < SPT; ++j)
+            issue3c(src, a.s, m, cxy[j], lz, zin && ((in_xy >> j) & 1u), pend[j]);
     };
     // static image values (U if MOVI, else I and |grad I|) of the products positions
     float pre0[PPT], pre1[PPT];
