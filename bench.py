@@ -527,6 +527,15 @@ def main():
         mv16.close()
         log(f"single-pose fp32 {extra['single_pose_fp32_evals_per_s']:.1f} evals/s, "
             f"batched fp16 {extra['batched_fp16_evals_per_s']:.1f} evals/s")
+        # the mutual-information metric (patchSize 0, 256 x 256 bins) on the same pair
+        pair.eval_mi(mats)
+        t = time.perf_counter()
+        nmi = None
+        for _ in range(3):
+            nmi = pair.eval_mi(mats)
+        extra["mi_evals_per_s"] = 3 * P / (time.perf_counter() - t)
+        extra["mi_at_true_pose"] = float(nmi[0])
+        log(f"mutual information {extra['mi_evals_per_s']:.1f} evals/s")
         t0 = time.time()
         w_ct = synth.config3()
         log(f"CT workload synthesised in {time.time() - t0:.1f}s")

@@ -373,3 +373,30 @@ def test_ct_int16_half_storage_is_lossless(engine, wct):
         assert abs(r16.lc2[k] - res.lc2) < LC2_TOL, (k, r16.lc2[k], res.lc2)
     for h in (p16, p32, fv, mv16, mv32):
         h.close()
+
+
+@pytest.mark.parametrize("r", [1, 2, 4, 5, 6, 7, 10])
+def test_2d_radius_classes_match_oracle(engine, w3, r):
+    """Every radius class of the 2D kernel (r <= 3, <= 6, <= 10), in both roles: the
+    ultrasound warped onto a static image (2D source) and the MR warped onto a static
+    ultrasound frame (3D source)."""
+    w = synth.config0(n=96, patch_radius=r)
+    p = _pair_for(engine, w)
+    M = p.pose_to_matrix(w.true_pose)
+    res, I, U, G = _oracle_eval(w, M)
+    assert res.n_valid > 100
+    assert abs(p.eval(M) - res.lc2) < LC2_TOL, r
+    p.close()
+    us = w3.fixed
+    sl = us.data[us.data.shape[0] // 2][None].copy()
+    fixed = synth.Image(sl, (us.dims[0], us.dims[1], 1), us.spacing,
+                        (us.origin[0], us.origin[1], us.origin[2] + us.spacing[2] * (us.dims[2] // 2)))
+    ws = synth.Workload("2dto3d", fixed, w3.moving, us.spacing, r, True, False, 2, w3.true_pose)
+    p = _pair_for(engine, ws)
+    M = p.pose_to_matrix(w3.true_pose)
+    res, I, U, G = _oracle_eval(ws, M, src_ndim=3, grid_ndim=2)
+    res2 = O.lc2_exact(U[0], I[0], O.sobel(I[0], "sep"), r)
+    assert np.array_equal(p.debug_fetch(E.STAGE_WARP_I, M), I)
+    if res2.n_valid:
+        assert abs(p.eval(M) - res2.lc2) < LC2_TOL, r
+    p.close()
