@@ -447,6 +447,11 @@ int lc2e_volume_upload(lc2e_engine* e, const void* host, lc2e_dtype dtype, const
     else if (storage == LC2E_F16) f16 = 1;
     else if (storage == LC2E_F32) f16 = 0;
     else return fail(LC2E_ERR_INVALID, "storage must be F16, F32 or AUTO");
+    // the samplers address a volume with 32-bit byte offsets
+    if (!f16 && n >= (1LL << 30))
+        return fail(LC2E_ERR_UNSUPPORTED,
+                    "volume of %lld voxels exceeds the 4 GiB that f32 storage can address; "
+                    "store it as F16", n);
     if (f16 && dtype == LC2E_F32) {
         // half storage must hold every value finitely: the samplers rely on finite voxels
         const float* h = static_cast<const float*>(host);

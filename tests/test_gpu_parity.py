@@ -241,6 +241,19 @@ def test_degenerate_constant_us(engine, w3):
     p.close()
 
 
+def test_upload_rejects_f32_volume_beyond_32_bit_offsets(engine):
+    """An f32 volume of 2^30 voxels would need byte offsets beyond 32 bits in the samplers:
+    refused before any allocation or read of the host buffer."""
+    import ctypes as C
+    g = E.make_geom((1024, 1024, 1024), (1, 1, 1), (0, 0, 0))
+    h = C.c_void_p()
+    small = np.zeros(16, np.float32)
+    rc = E.lib().lc2e_volume_upload(engine.h, small.ctypes.data_as(C.c_void_p), E.F32,
+                                    C.byref(g), E.F32, C.byref(h))
+    assert E.ERR.get(rc) == "UNSUPPORTED" and not h.value
+    assert b"F16" in E.lib().lc2e_last_error()
+
+
 def test_budget_rejects_oversized_pair(w3):
     e = E.Engine(0, hbm_budget=200_000)
     with pytest.raises(E.LC2Error) as ei:
