@@ -509,11 +509,12 @@ __device__ __forceinline__ void block_reduce_store(double am, double aw, double*
     }
 }
 
-// LDS row strides of the products planes and of the x-pass output: an odd number of
-// doubles, so that the 16 (or 32) lanes of one LDS access that walk down consecutive rows
-// fall into distinct bank pairs (an even stride such as 32 or 38 doubles puts every row
-// into the same few banks).
-__host__ __device__ constexpr int odd_stride(int n) { return n | 1; }
+// LDS row strides of the products planes and of the x-pass output: 2 (mod 4) doubles.
+// The x pass reads and writes each lane's row two doubles at a time (ds_read_b128 /
+// ds_write2_b64), 8 lanes per LDS cycle; with a stride of 4k + 2 doubles the 8 lanes that
+// walk down consecutive rows cover 8 distinct groups of 4 banks. (Odd strides suit
+// single-double accesses and left 33% more conflict cycles here.)
+__host__ __device__ constexpr int lds_stride(int n) { return n + ((6 - n % 4) % 4); }
 
 // Products of one position into the channel planes of the LDS buffer (plane size np,
 // position i). NC = 9 (+count): I, G, U, II, GG, IG, UU, UI, UG.
@@ -591,7 +592,7 @@ __device__ __forceinline__ void box_xy(const double* s_prod, const int* s_cnt, d
     static_assert(TX % 16 == 0 && TY % 16 == 0, "tile edges are multiples of 16");
     constexpr int NT = NC + (CNT ? 1 : 0);
     constexpr int SX = TX / 16;
-    constexpr int XS = odd_stride(TX + 1);
+    constexpr int XS = lds_stride(TX + 1);
     const int np = ps * py;
     for (int t = tid; t < NT * py * SX; t += NTHR) {
         const int row = t % py, rest = t / py;
@@ -617,7 +618,7 @@ __device__ __forceinline__ void box_xy(const double* s_prod, const int* s_cnt, d
 }
 
 // box_xy for a patch radius known only at run time (the 2D kernel), 1 <= r <= RMAX, over
-// the (TX + 2r) x (TY + 2r) products region of row stride odd_stride(TX + 2r): the strides
+// the (TX + 2r) x (TY + 2r) products region of row stride lds_stride(TX + 2r): the strides
 // become compile-time constants of each radius.
 template <int NC, bool CNT, int TX, int TY, int NTHR, int RMAX = kMaxR2, int RR = 1>
 __device__ __forceinline__ void box_xy_r(int r, const double* s_prod, const int* s_cnt,
@@ -625,7 +626,7 @@ __device__ __forceinline__ void box_xy_r(int r, const double* s_prod, const int*
                                          int tid) {
     if (r == RR)
         box_xy<NC, CNT, TX, TY, NTHR, 2 * RR + 1>(s_prod, s_cnt, s_xp, s_xcnt, s_out, s_ocnt,
-                                                  odd_stride(TX + 2 * RR), TY + 2 * RR, tid);
+                                                  lds_stride(TX + 2 * RR), TY + 2 * RR, tid);
     else if constexpr (RR < RMAX)
         box_xy_r<NC, CNT, TX, TY, NTHR, RMAX, RR + 1>(r, s_prod, s_cnt, s_xp, s_xcnt, s_out, s_ocnt,
                                                       tid);
@@ -646,8 +647,8 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
     constexpr int W = 2 * R + 1;
     constexpr int PX = TX + 2 * R, PY = TY + 2 * R;  // patch-support region
     constexpr int NP = PX * PY;
-    constexpr int PS = odd_stride(PX);               // LDS row stride of the products planes
-    constexpr int XS = odd_stride(TX + 1);           // LDS row stride of the x-pass output
+    constexpr int PS = lds_stride(PX);               // LDS row stride of the products planes
+    constexpr int XS = lds_stride(TX + 1);           // LDS row stride of the x-pass output
     constexpr int IXE = PX + 2, IYE = PY + 2;        // Sobel input region
     constexpr int NI = MOVI ? IXE * IYE : NP;        // warped positions per plane
     constexpr int SPT = (NI + NTHR - 1) / NTHR;      // warped samples per thread
@@ -915,8 +916,8 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
     constexpr int SPT = ((kTile + 2 * RMAX + 2) * (kTile + 2 * RMAX + 2) + kThreads - 1) / kThreads;
     constexpr int PPT = ((kTile + 2 * RMAX) * (kTile + 2 * RMAX) + kThreads - 1) / kThreads;
     const int R = a.r, W = 2 * R + 1, PX = kTile + 2 * R, NP = PX * PX, IXE = PX + 2;
-    const int PS = odd_stride(PX);                            // products row stride
-    constexpr int XS = odd_stride(kTile + 1);                 // x-pass row stride
+    const int PS = lds_stride(PX);                            // products row stride
+    constexpr int XS = lds_stride(kTile + 1);                 // x-pass row stride
     const int wex = MOVI ? IXE : PX;                          // warped region edge
     const int wofs = MOVI ? R + 1 : R;                        // it starts at x0 - wofs
     const int NI = wex * wex;
@@ -1455,7 +1456,7 @@ __global__ void k_convert(const void* in, int in_dtype, void* out, int out_f16, 
 constexpr size_t lds_bytes_2d(int r, bool movi) {
     const size_t nc = movi ? 7 : 9;
     const size_t PX = kTile + 2 * r, IXE = PX + 2;
-    const size_t PS = odd_stride((int)PX), XS = odd_stride(kTile + 1);
+    const size_t PS = lds_stride((int)PX), XS = lds_stride(kTile + 1);
     const size_t uni = std::max(2 * nc * PX * XS, movi ? IXE * IXE : PX * PX);
     return sizeof(double) * (nc * PX * PS + 8) + sizeof(float) * uni +
            (movi ? 0 : sizeof(int) * (PX * PS + PX * XS));
