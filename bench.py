@@ -111,9 +111,47 @@ def sweep_figures(eng, E, synth, steps=3):
            "sweep_lc2_at_true_pose": float(res.lc2[0])}
     log(f"sweep: {out['sweep_evals_per_s']:.1f} whole-sweep evals/s "
         f"({out['sweep_pixel_poses_per_s'] / 1e9:.2f} G pixel-poses/s)")
+    out.update(sweep_registration_figures(p, w, synth))
     p.close()
     fv.close()
     mv.close()
+    return out
+
+
+def sweep_registration_figures(p, w, synth, n_runs=3, seed=43):
+    """Rigid registrations of the 128-frame tracked sweep (6 DoF shared by all frames,
+    one matrix per frame) with the batched trust region (lc2reg_minimize_qmtr) on
+    -log(LC2), from the generating pose moved by up to 3 mm / 3 deg per parameter. Every
+    objective call evaluates its whole candidate set in one launch. Returns p50 / p99 wall
+    time, evaluations and the largest error against the generating pose."""
+    from lc2registration_amd import registration as REG
+    rng = np.random.default_rng(seed)
+    truth = np.asarray(w.true_pose, np.float64)[:6]
+    frames, centre = w.extra["frames"], w.extra["centre"]
+
+    def objective(xs):
+        mats = np.stack([synth.sweep_matrices(q, frames, centre) for q in xs])
+        l = p.eval_batch(mats).lc2
+        return np.where(l > 0, -np.log(np.maximum(l, 1e-300)), np.inf)
+
+    times, evals, errs = [], [], []
+    for _ in range(n_runs):
+        x0 = truth + rng.uniform(-3, 3, 6)
+        t = time.perf_counter()
+        x, _, nev = REG.minimize_qmtr(objective, x0, x0 - 10, x0 + 10, xtol=1e-3)
+        times.append(time.perf_counter() - t)
+        evals.append(nev)
+        errs.append(np.abs(x - truth))
+    ms = np.array(times) * 1e3
+    e = np.array(errs)
+    out = {"sweep_registration_p50_ms": float(np.percentile(ms, 50)),
+           "sweep_registration_p99_ms": float(np.percentile(ms, 99)),
+           "sweep_registration_evaluations": [int(v) for v in evals],
+           "sweep_registration_max_error_mm": float(e[:, :3].max()),
+           "sweep_registration_max_error_deg": float(e[:, 3:].max())}
+    log(f"sweep registration p50 {out['sweep_registration_p50_ms']:.0f} ms, {evals} evaluations, "
+        f"max error {out['sweep_registration_max_error_mm']:.3f} mm / "
+        f"{out['sweep_registration_max_error_deg']:.3f} deg")
     return out
 
 

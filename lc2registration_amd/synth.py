@@ -284,7 +284,7 @@ def sweep_matrices(pose, frames, centre):
     A = geo.affine_3d(np.r_[np.asarray(pose, np.float64)[:6], 1, 1, 1, 0, 0, 0], centre,
                       np.zeros(3)).astype(np.float64)
     A4 = np.vstack([A, [0, 0, 0, 1]])
-    return np.stack([(A4 @ F)[:3] for F in frames]).astype(np.float32)
+    return np.matmul(A4, np.asarray(frames, np.float64))[:, :3].astype(np.float32)
 
 
 def config2(seed=2, mr_dims=(320, 320, 256), n_frames=128, frame_dims=(512, 384),
