@@ -13,8 +13,8 @@
  * Admission is by bytes. Volumes and pairs are created through lc2e.h against the engine's
  * HBM budget (LC2E_ERR_BUDGET when they do not fit), and so are the slots' staging buffers.
  * If a merged launch does not fit the budget, its requests are evaluated one by one.
- * Pairs are served in the order of their oldest queued request; a pair has at most one
- * launch in flight. Each request's values are those of lc2e_eval_batch on its own poses:
+ * Pairs are served in the order of their oldest queued request, and a pair's requests are
+ * launched in the order they arrived; a pair has at most one launch in flight. Each request's values are those of lc2e_eval_batch on its own poses:
  * they do not depend on what it was merged with, and its status is its own.
  * While a server runs, the pairs it serves must not be evaluated through lc2e.h (the
  * engine is not re-entrant); other pairs and volumes may be created and destroyed. A pair

@@ -193,29 +193,31 @@ struct lc2s_server {
     // Under the lock: the poses take() would put into one launch of the pair.
     int peek_total(lc2e_pair* pair) const {
         int total = 0;
-        bool first = true;
         for (const auto& r : queue) {
             if (r->pair != pair) continue;
-            if (first || total + r->n <= max_poses) total += r->n;
-            first = false;
+            if (total > 0 && total + r->n > max_poses) break;
+            total += r->n;
         }
         return total;
     }
 
-    // Under the lock: move the pair's queued requests, in order, into the slot while they
-    // fit one launch (a request larger than max_poses goes alone).
+    // Under the lock: move the pair's oldest queued requests, in arrival order, into the
+    // slot while they fit one launch (a request larger than max_poses goes alone). The
+    // first request that does not fit ends the launch, so a pair's requests are launched
+    // in the order they arrived.
     void take(Slot& s, lc2e_pair* pair) {
         s.batch.clear();
         s.total = 0;
         for (auto it = queue.begin(); it != queue.end();) {
             Request& r = **it;
-            if (r.pair == pair && (s.batch.empty() || s.total + r.n <= max_poses)) {
-                s.total += r.n;
-                s.batch.push_back(*it);
-                it = queue.erase(it);
-            } else {
+            if (r.pair != pair) {
                 ++it;
+                continue;
             }
+            if (!s.batch.empty() && s.total + r.n > max_poses) break;
+            s.total += r.n;
+            s.batch.push_back(*it);
+            it = queue.erase(it);
         }
         s.pair = pair;
         s.busy = true;
