@@ -413,3 +413,26 @@ def test_2d_radius_classes_match_oracle(engine, w3, r):
     if res2.n_valid:
         assert abs(p.eval(M) - res2.lc2) < LC2_TOL, r
     p.close()
+
+
+def test_autotune_keeps_results_and_caches(engine, w3, tmp_path):
+    """The z-chunk autotuner picks a multiple of 8 planes, leaves LC2 bit-identical, and its
+    JSON cache answers the second call without timing."""
+    from lc2registration_amd import autotune
+    p = _pair_for(engine, w3)
+    rng = np.random.default_rng(12)
+    mats = np.stack([p.pose_to_matrix(synth.perturb(w3.true_pose, rng, 3)) for _ in range(6)])
+    before = p.eval_batch(mats).lc2
+    cache = str(tmp_path / "tune.json")
+    z = autotune.tune(p, mats, iters=2, path=cache)
+    assert z > 0 and z % 8 == 0
+    assert np.array_equal(p.eval_batch(mats).lc2, before)
+    assert p.get_tuning() == z            # the chunk the last launch used
+    entry = autotune.load(cache)
+    assert len(entry) == 1 and list(entry.values())[0]["zchunk"] == z
+    p2 = _pair_for(engine, w3)
+    assert autotune.tune(p2, mats, iters=2, path=cache) == z      # from the cache
+    assert np.array_equal(p2.eval_batch(mats).lc2, before)
+    assert p2.get_tuning() == z
+    p.close()
+    p2.close()
