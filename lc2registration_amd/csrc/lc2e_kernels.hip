@@ -416,8 +416,10 @@ __device__ __forceinline__ double explained_var(double a, double b, double c, do
         return (b * p * p - 2.0 * c * p * q + a * q * q) * rcp64(det);
     const double tr = a + b;
     if (tr > 0.0) {
-        const double u = sqrt(a) * p + (c < 0.0 ? -1.0 : 1.0) * sqrt(b) * q;
-        return (u * u) / (tr * tr);
+        // sqrt and division through the refined hardware estimates (a few ulp), as above
+        const double sa = a > 0.0 ? a * rsq64(a) : 0.0, sb = b > 0.0 ? b * rsq64(b) : 0.0;
+        const double u = sa * p + (c < 0.0 ? -1.0 : 1.0) * sb * q;
+        return (u * u) * rcp64(tr * tr);
     }
     return 0.0;
 }
