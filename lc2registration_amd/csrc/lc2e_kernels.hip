@@ -915,24 +915,25 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
     constexpr bool CNT = !MOVI;
     // per-thread gathers / products positions at the largest radius of this variant
     // (r <= RMAX): the pipelined gathers are held in registers
-    constexpr int SPT = ((kTile + 2 * RMAX + 2) * (kTile + 2 * RMAX + 2) + kThreads - 1) / kThreads;
-    constexpr int PPT = ((kTile + 2 * RMAX) * (kTile + 2 * RMAX) + kThreads - 1) / kThreads;
-    const int R = a.r, W = 2 * R + 1, PX = kTile + 2 * R, NP = PX * PX, IXE = PX + 2;
+    constexpr int SPT = ((kTile2X + 2 * RMAX + 2) * (kTile2Y + 2 * RMAX + 2) + kThreads - 1) / kThreads;
+    constexpr int PPT = ((kTile2X + 2 * RMAX) * (kTile2Y + 2 * RMAX) + kThreads - 1) / kThreads;
+    const int R = a.r, W = 2 * R + 1, PX = kTile2X + 2 * R, PY = kTile2Y + 2 * R, NP = PX * PY;
+    const int IXE = PX + 2, IYE = PY + 2;
     const int PS = lds_stride(PX);                            // products row stride
-    constexpr int XS = lds_stride(kTile + 1);                 // x-pass row stride
+    constexpr int XS = lds_stride(kTile2X + 1);               // x-pass row stride
     const int wex = MOVI ? IXE : PX;                          // warped region edge
     const int wofs = MOVI ? R + 1 : R;                        // it starts at x0 - wofs
-    const int NI = wex * wex;
+    const int NI = MOVI ? IXE * IYE : NP;
     // The x-pass output s_xp shares its space with the warped tile s_in, which is dead
     // once the products are stored, before the box sums start; the next frame's warped
     // tile is written only after the box sums and the combine.
-    const int uni = max(2 * NC * PX * XS, NI);                // shared region, in floats
-    double* s_prod = s_dyn;                                   // NC*PX*PS
-    double* s_red = s_prod + NC * PX * PS;                    // 8
-    double* s_xp = s_red + 8;                                 // NC*PX*XS
+    const int uni = max(2 * NC * PY * XS, NI);                // shared region, in floats
+    double* s_prod = s_dyn;                                   // NC*PY*PS
+    double* s_red = s_prod + NC * PY * PS;                    // 8
+    double* s_xp = s_red + 8;                                 // NC*PY*XS
     float* s_in = reinterpret_cast<float*>(s_xp);             // NI
-    int* s_cnt = reinterpret_cast<int*>(s_in + uni);          // PX*PS
-    int* s_xcnt = s_cnt + PX * PS;                            // PX*XS
+    int* s_cnt = reinterpret_cast<int*>(s_in + uni);          // PY*PS
+    int* s_xcnt = s_cnt + PY * PS;                            // PY*XS
 
     int tile, pose;
     if (!map_block(a.tiles, a.npose, tile, pose)) return;  // whole work-group
@@ -941,7 +942,7 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
     const int tcol = tile % cols, run = tile / cols;
     const int f0 = run * a.zchunk, f1 = min(f0 + a.zchunk, a.v.Z);
     const int tx = tcol % a.ntx, ty = tcol / a.ntx;
-    const int x0 = R + tx * kTile, y0 = R + ty * kTile;
+    const int x0 = R + tx * kTile2X, y0 = R + ty * kTile2Y;
     const int X = a.v.X, Y = a.v.Y;
     const float lz = a.v.oz;
     const T* __restrict__ src = static_cast<const T*>(a.src);
@@ -1003,7 +1004,7 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
         }
     };
     // static ultrasound statistics of this thread's output pixel
-    const int ox = tid % kTile, oy = tid / kTile;
+    const int ox = tid % kTile2X, oy = tid / kTile2X;
     const bool out_ok = (x0 + ox < X - R) && (y0 + oy < Y - R);
     double mu_pre = 0.0, cyy_pre = -1.0;
     auto issue_stats = [&](int frame) {
@@ -1069,20 +1070,20 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
                     I = cur0[j];
                     G = cur1[j];
                 }
-                store_products<NC>(s_prod, s_cnt, PX * PS, p_prod_ofs[j], I, G, U);
+                store_products<NC>(s_prod, s_cnt, PY * PS, p_prod_ofs[j], I, G, U);
             }
         }
         __syncthreads();  // s_in reads done; products visible
         // the next frame's gathers fly during the box sums and the combine
         if (f + 1 < f1) issue_frame(f + 1);
         // 3. box sums
-        box_xy_r<NC, CNT, kTile, kTile, kThreads, RMAX>(R, s_prod, s_cnt, s_xp, s_xcnt, s_prod,
+        box_xy_r<NC, CNT, kTile2X, kTile2Y, kThreads, RMAX>(R, s_prod, s_cnt, s_xp, s_xcnt, s_prod,
                                                         s_cnt, tid);
         // 4. combine; each thread adds its pixel's values over the run's frames in order
         if (out_ok) {
             double S[NC];
 #pragma unroll
-            for (int c = 0; c < NC; ++c) S[c] = s_prod[c * kTile * kTile + tid];
+            for (int c = 0; c < NC; ++c) S[c] = s_prod[c * kTile2X * kTile2Y + tid];
             double w, meas;
             bool ok;
             if (MOVI) ok = combine_static_u(S, mu, cyy, W * W, w, meas);
@@ -1457,11 +1458,11 @@ __global__ void k_convert(const void* in, int in_dtype, void* out, int out_f16, 
 // image, 9 plus an integer count for a moving ultrasound).
 constexpr size_t lds_bytes_2d(int r, bool movi) {
     const size_t nc = movi ? 7 : 9;
-    const size_t PX = kTile + 2 * r, IXE = PX + 2;
-    const size_t PS = lds_stride((int)PX), XS = lds_stride(kTile + 1);
-    const size_t uni = std::max(2 * nc * PX * XS, movi ? IXE * IXE : PX * PX);
-    return sizeof(double) * (nc * PX * PS + 8) + sizeof(float) * uni +
-           (movi ? 0 : sizeof(int) * (PX * PS + PX * XS));
+    const size_t PX = kTile2X + 2 * r, PY = kTile2Y + 2 * r, IXE = PX + 2, IYE = PY + 2;
+    const size_t PS = lds_stride((int)PX), XS = lds_stride(kTile2X + 1);
+    const size_t uni = std::max(2 * nc * PY * XS, movi ? IXE * IYE : PX * PY);
+    return sizeof(double) * (nc * PY * PS + 8) + sizeof(float) * uni +
+           (movi ? 0 : sizeof(int) * (PY * PS + PY * XS));
 }
 static_assert(lds_bytes_2d(kMaxR2, false) <= 160 * 1024, "2D fused kernel exceeds the 160 KiB of LDS");
 
@@ -1544,8 +1545,8 @@ int fused_partials_3d(const VGeom& v, int r) {
 int fused_blocks_2d(const VGeom& v, int r, int* ntx, int* nty) {
     const int ex = v.X - 2 * r, ey = v.Y - 2 * r;
     if (ex <= 0 || ey <= 0) { *ntx = *nty = 0; return 0; }
-    *ntx = (ex + kTile - 1) / kTile;
-    *nty = (ey + kTile - 1) / kTile;
+    *ntx = (ex + kTile2X - 1) / kTile2X;
+    *nty = (ey + kTile2Y - 1) / kTile2Y;
     return *ntx * *nty * v.Z;
 }
 

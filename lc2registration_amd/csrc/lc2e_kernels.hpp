@@ -8,8 +8,9 @@
 
 namespace lc2e {
 
-constexpr int kTile = 16;      // output tile edge (x and y) of the 2D fused kernel
-constexpr int kThreads = 256;  // work-group size of the 2D fused kernel (16 x 16 outputs)
+constexpr int kTile2X = 16;    // output tile of the 2D fused kernel: kTile2X x kTile2Y,
+constexpr int kTile2Y = 16;    // one thread per output
+constexpr int kThreads = kTile2X * kTile2Y;  // work-group size of the 2D fused kernel
 constexpr int kTile3X = 32;    // output tile of the 3D fused kernel: 32 x 16 columns,
 constexpr int kTile3Y = 16;    // one work-group of 512 threads, one column per thread
 // Resident volumes carry this many zeroed bytes past their last voxel (paired x loads).
