@@ -187,3 +187,29 @@ def test_kidney_2d_yaml_registration_reaches_reference_optimum(reg, tmp_path):
     assert np.allclose(x[3:], [1, 1, 0])
     assert lc2 > 0.255
     assert os.path.exists(tmp_path / "parameters2d_result.yaml")
+
+
+def test_cli_registers_the_kidney_phantom(tmp_path):
+    """The command-line tool (`lc2_registration -m 2D -c cfg.yaml`, the reference's
+    `CMS_RegistrationTools` invocation) runs the same registration as a process of its own:
+    exit status 0, and a result YAML with IsRegistered and the reference's optimum."""
+    import subprocess
+    import sys
+    from lc2registration_amd import engine as E
+    exe = os.path.join(os.path.dirname(E.library_path()), "lc2_registration")
+    assert os.path.exists(exe)
+    cfg = str(tmp_path / "parameters2d.yaml")
+    man = _kidney_config(cfg, tmp_path)
+    ref = np.array(man["rigid_dumps"]["moving.bin"]["pose_tx_ty_rdeg"])
+    r = subprocess.run([exe, "-m", "2D", "-c", cfg, "--avoid-igtl", "--no-images"],
+                       capture_output=True, text=True, timeout=120)
+    sys.stdout.write(r.stdout[-400:])
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = yaml.safe_load(open(tmp_path / "parameters2d_result.yaml"))
+    assert out["Outputs"]["IsRegistered"] is True
+    v = out["InputParameters"]["OptimizerSettings"]["InitialValues"]
+    assert abs(v["TX"] - ref[0]) < 0.3 and abs(v["TY"] - ref[1]) < 0.3 and abs(v["R"] - ref[2]) < 0.3
+    bad = subprocess.run([exe, "-m", "2D"], capture_output=True, text=True, timeout=60)
+    assert bad.returncode == 2 and "usage" in bad.stdout        # no configuration given
+    bad = subprocess.run([exe, "-m", "4D", "-c", cfg], capture_output=True, text=True, timeout=60)
+    assert bad.returncode != 0 and bad.stderr.strip()
