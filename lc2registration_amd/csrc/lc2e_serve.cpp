@@ -142,25 +142,35 @@ struct lc2s_server {
     }
 
     // Staging for `floats` matrix floats and `n` poses (the slot is idle). Device bytes are
-    // charged to the engine's HBM budget.
+    // charged to the engine's HBM budget. On failure the slot holds no staging at all, so
+    // the next call allocates again.
     int reserve(Slot& s, size_t floats, int n) {
         if (floats <= s.mats_cap && n <= s.out_cap) return LC2E_OK;
         const size_t fc = std::max(floats, s.mats_cap);
         const int nc = std::max(n, s.out_cap);
         free_buffers(s);
         int rc = lc2e::engine_alloc(engine, (void**)&s.d_mats, fc * sizeof(float));
-        if (rc < 0) return rc;
-        s.mats_cap = fc;
-        rc = lc2e::engine_alloc(engine, (void**)&s.d_out, (size_t)nc * 3 * sizeof(double));
-        if (rc < 0) return rc;
-        s.out_cap = nc;
-        if (hipHostMalloc((void**)&s.h_mats, fc * sizeof(float), hipHostMallocDefault) != hipSuccess ||
-            hipHostMalloc((void**)&s.h_out, (size_t)nc * 3 * sizeof(double), hipHostMallocDefault) !=
-                hipSuccess) {
-            lc2e::set_last_error("serving: pinned staging allocation failed");
-            return LC2E_ERR_HIP;
+        if (rc < 0) {
+            s.d_mats = nullptr;
+        } else {
+            s.mats_cap = fc;
+            rc = lc2e::engine_alloc(engine, (void**)&s.d_out, (size_t)nc * 3 * sizeof(double));
+            if (rc < 0) s.d_out = nullptr;
+            else s.out_cap = nc;
         }
-        return LC2E_OK;
+        if (rc >= 0) {
+            void* hm = nullptr;
+            void* ho = nullptr;
+            if (hipHostMalloc(&hm, fc * sizeof(float), hipHostMallocDefault) == hipSuccess) s.h_mats = (float*)hm;
+            if (hipHostMalloc(&ho, (size_t)nc * 3 * sizeof(double), hipHostMallocDefault) == hipSuccess)
+                s.h_out = (double*)ho;
+            if (!s.h_mats || !s.h_out) {
+                lc2e::set_last_error("serving: pinned staging allocation failed");
+                rc = LC2E_ERR_HIP;
+            }
+        }
+        if (rc < 0) free_buffers(s);
+        return rc;
     }
 
     int queued_poses(lc2e_pair* p) const {

@@ -33,6 +33,7 @@ thread_local std::string g_err;
 size_t g_budget = 0, g_used = 0;  // bytes, 0 = unlimited
 int g_query_delay = 0;            // "not ready" answers before an event completes
 int g_budget_poses = 0;           // > 0: launches of more poses fail with LC2E_ERR_BUDGET
+int g_host_fail_at = -1;          // >= 0: that many pinned allocations succeed, the next fails
 struct Launch {
     int pair, n;
     std::vector<float> first;  // first float of each pose's matrices
@@ -87,6 +88,10 @@ int fake_launch(int k, int* pair, int* n, float* first, int cap) {
 int fake_max_in_flight() {
     std::lock_guard<std::mutex> lk(g_mu);
     return g_max_in_flight;
+}
+void fake_fail_host_malloc(int after) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_host_fail_at = after;
 }
 long long fake_used() {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -200,6 +205,13 @@ hipError_t hipMemcpyAsync(void* dst, const void* src, size_t bytes, hipMemcpyKin
     return hipSuccess;
 }
 hipError_t hipHostMalloc(void** p, size_t bytes, unsigned int) {
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (g_host_fail_at >= 0 && g_host_fail_at-- == 0) {
+            *p = nullptr;
+            return hipErrorOutOfMemory;
+        }
+    }
     *p = std::malloc(bytes ? bytes : 1);
     return *p ? hipSuccess : hipErrorOutOfMemory;
 }

@@ -23,6 +23,7 @@ int fake_launches();
 int fake_launch(int k, int* pair, int* n, float* first, int cap);
 int fake_max_in_flight();
 long long fake_used();
+void fake_fail_host_malloc(int after);
 }
 
 #define CHECK(cond)                                                                  \
@@ -223,6 +224,34 @@ void scenario_destroy_with_work() {
     std::printf("destroy_with_work ok\n");
 }
 
+void scenario_staging_failure() {
+    // the second pinned staging allocation of a slot fails: that launch's request fails,
+    // the slot keeps no half-made staging, and the next request allocates it again
+    fake_config(2, 0, 0);
+    lc2e_engine* e = fake_engine();
+    lc2e_pair* a = fake_pair(1, 1, 0.1, 0);
+    lc2s_server* s = nullptr;
+    CHECK(lc2s_create(e, 8, 1, 0.0, &s) == LC2E_OK);
+    fake_fail_host_malloc(1);
+    std::vector<float> m = poses(2, 1, 5.0f);
+    long long t;
+    double l[2];
+    CHECK(lc2s_submit(s, a, 2, m.data(), &t) == LC2E_OK);
+    CHECK(lc2s_wait(s, t, -1.0, l, nullptr, nullptr) == LC2E_ERR_HIP);
+    CHECK(std::strstr(lc2e_last_error(), "pinned staging") != nullptr);
+    CHECK(fake_used() == 0);  // the device staging went back too
+    for (int k = 0; k < 2; ++k) {
+        CHECK(lc2s_submit(s, a, 2, m.data(), &t) == LC2E_OK);
+        CHECK(lc2s_wait(s, t, -1.0, l, nullptr, nullptr) == LC2E_OK);
+        CHECK(l[0] == expect_lc2(m.data(), 1) && l[1] == expect_lc2(m.data() + 12, 1));
+    }
+    lc2s_destroy(s);
+    CHECK(fake_used() == 0);
+    fake_free_pair(a);
+    fake_free_engine(e);
+    std::printf("staging_failure ok\n");
+}
+
 void scenario_bad_arguments() {
     lc2e_engine* e = fake_engine();
     lc2s_server* s = nullptr;
@@ -248,6 +277,7 @@ int main() {
     scenario_errors_and_budget();
     scenario_admission_by_share();
     scenario_destroy_with_work();
+    scenario_staging_failure();
     scenario_bad_arguments();
     std::printf("all serving scenarios passed\n");
     return 0;
