@@ -262,11 +262,21 @@ int ensure_capacity(lc2e_pair* p, int n, int bpp) {
         p->d_partials = nullptr;
         return rc;
     }
-    HIP_TRY(hipHostMalloc((void**)&p->h_mats, (size_t)cap * mpp * 12 * sizeof(float),
-                          hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc((void**)&p->h_out, (size_t)cap * 3 * sizeof(double), hipHostMallocDefault));
-    p->cap = cap;
+    p->cap = cap;  // free_batch returns the device bytes by these sizes
     p->part_bpp = bpp;
+    void* hm = nullptr;
+    void* ho = nullptr;
+    hipError_t he = hipHostMalloc(&hm, (size_t)cap * mpp * 12 * sizeof(float), hipHostMallocDefault);
+    if (he == hipSuccess) {
+        p->h_mats = (float*)hm;
+        he = hipHostMalloc(&ho, (size_t)cap * 3 * sizeof(double), hipHostMallocDefault);
+        if (he == hipSuccess) p->h_out = (double*)ho;
+    }
+    if (he != hipSuccess) {
+        free_batch(p);  // no half-made buffers: the next call allocates again
+        return fail(LC2E_ERR_HIP, "pinned staging allocation of %d poses failed: %s", cap,
+                    hipGetErrorString(he));
+    }
     return LC2E_OK;
 }
 
@@ -329,10 +339,13 @@ int run_batch(lc2e_pair* p, int n, const Tiling& t) {
             int rc = enqueue_batch(p, n, t, st);
             hipGraph_t graph = nullptr;
             hipError_t ce = hipStreamEndCapture(st, &graph);
-            if (rc < 0) return rc;
-            if (ce != hipSuccess) return fail(LC2E_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ce));
+            if (rc >= 0 && ce == hipSuccess) ce = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            if (rc < 0 || ce != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                if (rc < 0) return rc;
+                return fail(LC2E_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ce));
+            }
             g.graph = graph;
-            HIP_TRY(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
             it = p->graphs.emplace(n, g).first;
         }
         HIP_TRY(hipGraphLaunch(it->second.exec, st));
@@ -730,6 +743,7 @@ int lc2e_sync(lc2e_engine* e) {
 
 int lc2e_debug_fetch(lc2e_pair* p, lc2e_stage stage, const float* m12, void* host) {
     if (!p || !host || !m12) return fail(LC2E_ERR_INVALID, "null argument");
+    HIP_TRY(hipSetDevice(p->e->device));
     hipStream_t st = p->e->stream;
     const long long V = (long long)p->v.X * p->v.Y * p->v.Z;
     const int mpp = p->mats_per_pose();
@@ -802,6 +816,7 @@ int lc2e_pair_set_tuning_n(lc2e_pair* p, int n, int zchunk) {
 int lc2e_pair_autotune(lc2e_pair* p, int n, const float* mats, int iters, int* best_zchunk,
                        double* best_ms) {
     if (!p || !mats || n <= 0 || iters <= 0) return fail(LC2E_ERR_INVALID, "invalid argument");
+    HIP_TRY(hipSetDevice(p->e->device));
     if (p->info.grid_ndim != 3) {
         if (best_zchunk) *best_zchunk = 1;
         if (best_ms) *best_ms = 0.0;
