@@ -76,10 +76,11 @@ def cpu_baseline(w, M, budget_s):
             "exact_fp64_evals_per_s": out["exact_fp64"][0]}
 
 
-def sweep_figures(eng, E, synth, steps=3):
+def sweep_figures(eng, E, synth, steps=3, registration=True):
     """Tracked 2D sweep against an MR volume (2D-to-3D): 128 uint8 frames of 512x384
     (0.3 mm) against MR 320x320x256 stored f16, patch 7x7, 64 candidate poses per launch,
-    one matrix per frame and pose. Returns whole-sweep evaluations/s and pixel-poses/s."""
+    one matrix per frame and pose. Returns whole-sweep evaluations/s and pixel-poses/s, and
+    with `registration` the rigid sweep registration figures as well."""
     t0 = time.time()
     w = synth.config2()
     log(f"sweep workload synthesised in {time.time() - t0:.1f}s")
@@ -111,7 +112,8 @@ def sweep_figures(eng, E, synth, steps=3):
            "sweep_lc2_at_true_pose": float(res.lc2[0])}
     log(f"sweep: {out['sweep_evals_per_s']:.1f} whole-sweep evals/s "
         f"({out['sweep_pixel_poses_per_s'] / 1e9:.2f} G pixel-poses/s)")
-    out.update(sweep_registration_figures(p, w, synth))
+    if registration:
+        out.update(sweep_registration_figures(p, w, synth))
     p.close()
     fv.close()
     mv.close()
@@ -468,7 +470,8 @@ def main():
     if args.sweep_only:
         from lc2registration_amd import engine as E
         from lc2registration_amd import synth
-        print(json.dumps(sweep_figures(E.Engine(0), E, synth, steps=args.steps)), flush=True)
+        print(json.dumps(sweep_figures(E.Engine(0), E, synth, steps=args.steps,
+                                       registration=False)), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1",
@@ -522,13 +525,21 @@ def main():
     alg_bytes = P * V * ALG_BYTES_PER_VOXEL
     achieved = alg_bytes / avg_kernel_s / 1e9
     t_valu = P * V * (ALG_F32_PER_VOXEL / FP32_PEAK + ALG_F64_PER_VOXEL / FP64_PEAK)
-    traffic, valu_insts = None, None
+    traffic, valu_insts, tr = None, None, {}
     if os.path.exists(PROFILE_TRAFFIC):
         with open(PROFILE_TRAFFIC) as fh:
             tr = json.load(fh)
         if tr.get("poses") == P:
             traffic = tr.get("hbm_bytes_per_launch")
             valu_insts = tr.get("sq_insts_valu_per_launch")
+        else:
+            tr = {}
+
+    def ratio(num, den, complement=False):
+        a, b = tr.get(num), tr.get(den)
+        if not a or not b:
+            return None
+        return 1.0 - a / b if complement else a / b
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "kernel": "k_fused3d<R=3, moving high-res>",
@@ -541,7 +552,15 @@ def main():
                 # (profiles/), 4 cycles per wave64 instruction on a 16-lane SIMD, over
                 # 1024 SIMDs at 2.4 GHz
                 "valu_issue_frac": (valu_insts * 4 / (SIMDS * CLOCK_HZ) / avg_kernel_s
-                                    if valu_insts else None)}
+                                    if valu_insts else None),
+                # cache behaviour of the profiled launch (TCC / TCP / SQ counters)
+                "l2_hit_rate": (tr["tcc_hit_per_launch"] / (tr["tcc_hit_per_launch"] +
+                                                            tr["tcc_miss_per_launch"])
+                                if tr.get("tcc_hit_per_launch") else None),
+                "vl1d_hit_rate": ratio("tcp_tcc_read_req_per_launch",
+                                       "tcp_total_cache_accesses_per_launch", complement=True),
+                "lds_conflict_cycle_frac": ratio("sq_lds_bank_conflict_per_launch",
+                                                 "sq_active_inst_lds_per_launch")}
 
     extra = {}
     if not args.no_extra:
