@@ -131,7 +131,11 @@ int lc2e_eval_batch(lc2e_pair* p, int n, const float* mats, double* lc2, double*
 
 /* Stream-ordered form for serving loops: d_mats and d_out are DEVICE pointers,
  * d_out receives 3 doubles per pose (sum_measure, sum_w, lc2). `stream` is a hipStream_t
- * (NULL = the engine's stream). Nothing is synchronised. */
+ * (NULL = the engine's stream). Nothing is synchronised. The pair's partial-sum buffer is
+ * shared by its launches: two launches of ONE pair must not overlap (enqueue them on one
+ * stream, or wait for the first); launches of different pairs may run concurrently. A
+ * batch larger than any before it on this pair grows that buffer first (hipFree and
+ * hipMalloc, which wait for the device). */
 int lc2e_eval_batch_async(lc2e_pair* p, int n, const float* d_mats, double* d_out, void* stream);
 int lc2e_sync(lc2e_engine* e);
 
