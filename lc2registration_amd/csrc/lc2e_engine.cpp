@@ -678,7 +678,7 @@ int lc2e_eval_batch_async(lc2e_pair* p, int n, const float* d_mats, double* d_ou
         HIP_TRY(lc2e::launch_fused(a, p->info.grid_ndim, p->info.src_ndim,
                                    p->info.high_res_is_moving, n, st));
     }
-    HIP_TRY(lc2e::launch_reduce(p->d_partials, t.bpp, n, d_out, st));
+    HIP_TRY(lc2e::launch_reduce(p->d_partials, t.tiles > 0 ? t.bpp : 0, n, d_out, st));
     return LC2E_OK;
 }
 

@@ -436,3 +436,90 @@ def test_autotune_keeps_results_and_caches(engine, w3, tmp_path):
     assert p2.get_tuning() == z
     p.close()
     p2.close()
+
+
+# --- edge cases of the tiling --------------------------------------------------------------
+
+@pytest.mark.parametrize("moving_high_res", [True, False])
+def test_3d_ragged_grid_matches_oracle(engine, moving_high_res):
+    """A grid far from the tile and chunk sizes: 29 x 13 x 7 patch centres at r = 3 (one
+    partial 32 x 16 tile column, fewer planes than one partial-sum block), in both roles."""
+    w = synth.config1(seed=11, mr_dims=(40, 36, 30), us_dims=(35, 19, 13), us_spacing=1.0,
+                      patch_radius=3)
+    if not moving_high_res:
+        w = synth.Workload("ragged_swap", w.moving, w.fixed, (1.0, 1.0, 1.0), 3, False, True, 3,
+                           np.r_[0, 0, 0, 0, 0, 0, 1, 1, 1, 0, 0, 0])
+    p = _pair_for(engine, w)
+    if moving_high_res:
+        assert tuple(p.virt_dims) == (35, 19, 13)
+    rng = np.random.default_rng(4)
+    poses = [w.true_pose, synth.perturb(w.true_pose, rng, 3, 0.5)]
+    mats = np.stack([p.pose_to_matrix(q) for q in poses])
+    b = p.eval_batch(mats)
+    for k, M in enumerate(mats):
+        res = _oracle_eval(w, M)[0]
+        assert res.n_valid > 50
+        assert abs(b.lc2[k] - res.lc2) < LC2_TOL, (k, b.lc2[k], res.lc2)
+        assert abs(b.sum_w[k] - res.sum_w) < 1e-9 * res.sum_w
+    p.close()
+
+
+def test_grids_without_a_patch_centre_are_degenerate(engine):
+    """No patch fits the grid (3D: 4 planes at r = 2; 2D: 4 columns at r = 2): every sum is
+    0, LC2 is 0 and the status says degenerate, for one pose and for a batch."""
+    rng = np.random.default_rng(2)
+    M = np.eye(3, 4, dtype=np.float32).ravel()  # identity pose
+    for us_shape, mr_shape, dim in (((4, 12, 12), (20, 20, 20), 3), ((30, 4), (20, 20), 2)):
+        sp, org = (1.0,) * dim, (0.0,) * dim
+        us = rng.integers(1, 255, us_shape).astype(np.uint8)
+        mr = rng.uniform(1, 100, mr_shape).astype(np.float32)
+        fv = engine.upload(us, sp, org)
+        mv = engine.upload(mr, sp, org)
+        p = engine.pair(fv, mv, sp, True, False, 2, dim)
+        b = p.eval_batch(np.stack([M, M, M]))
+        assert b.status == E.OK_DEGENERATE
+        assert np.all(b.lc2 == 0.0) and np.all(b.sum_w == 0.0) and np.all(b.sum_measure == 0.0)
+        assert p.eval(M) == 0.0
+        p.close()
+        fv.close()
+        mv.close()
+
+
+def test_sweep_with_partial_last_frame_run(engine):
+    """18 tracked frames: one full run of 16 frames and a partial run of 2. The sums equal
+    the oracle's per-frame sums; one pose alone equals the same pose in a batch. At 0.8 mm
+    the identity resampling of a frame into the virtual grid is not exact (x * 0.8 / 0.8
+    rounds), so U is the oracle's resampled frame, as in the reference, not the raw one."""
+    w = synth.config2(seed=5, mr_dims=(64, 64, 56), n_frames=18, frame_dims=(40, 24),
+                      frame_spacing=0.8, sweep_mm=24.0, patch_radius=2)
+    f, m = w.fixed, w.moving
+    fv = engine.upload(f.data, f.spacing, f.origin)
+    mv = engine.upload(m.data, m.spacing, m.origin, E.F16)
+    p = engine.pair(fv, mv, w.resampled_spacing, True, False, 2, 2)
+    p.set_per_frame(True)
+    rng = np.random.default_rng(6)
+    poses = [w.true_pose, w.true_pose + np.r_[rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 3)]]
+    mats = np.stack([synth.sweep_matrices(q, w.extra["frames"], w.extra["centre"]) for q in poses])
+    b = p.eval_batch(mats)
+    mr16 = m.data.astype(np.float16).astype(np.float32)
+    mg = O.geom(m.dims, m.spacing, m.origin)
+    X, Y = f.dims[0], f.dims[1]
+    vg = O.geom((X, Y, 1), (f.spacing[0], f.spacing[1], 1.0), (0, 0, 0))
+    fg = O.geom((X, Y, 1), (f.spacing[0], f.spacing[1], 1.0), (f.origin[0], f.origin[1], 0))
+    Us = [O.warp_2d(f.data[k].astype(np.float32), fg, vg, [1, 0, 0, 0, 1, 0]) for k in range(18)]
+    assert np.array_equal(p.debug_fetch(E.STAGE_WARP_US, mats[0]), np.stack(Us))
+    for i in range(len(poses)):
+        sm = sw = 0.0
+        for k in range(18):
+            I = O.warp_3d(mr16, mg, vg, mats[i, k].ravel())[0]
+            res = O.lc2_exact(Us[k], I, O.sobel(I, "sep"), 2)
+            sm += res.sum_measure
+            sw += res.sum_w
+        assert sw > 0
+        assert abs(b.sum_w[i] - sw) < 1e-9 * sw, i
+        assert abs(b.sum_measure[i] - sm) < 1e-8 * sm, i
+        one = p.eval_batch(mats[i][None])
+        assert one.lc2[0] == b.lc2[i] and one.sum_w[0] == b.sum_w[i]
+    p.close()
+    fv.close()
+    mv.close()
