@@ -790,10 +790,11 @@ Optimizers3D::Optimizers3D(const LC2Configuration3D* c) : OptimizerBase(12, 4) {
 
 void Optimizers3D::GetOptimalTransform(float* out) {
     static_cast<Interpolator3D*>(interp_)->GetMovingToFixedTransform(
-        make_float3(OptimalSolution[3], OptimalSolution[4], OptimalSolution[5]),
-        make_float3(OptimalSolution[0], OptimalSolution[1], OptimalSolution[2]),
-        make_float3(OptimalSolution[6], OptimalSolution[7], OptimalSolution[8]),
-        make_float3(OptimalSolution[9], OptimalSolution[10], OptimalSolution[11]), out);
+        make_float3((float)OptimalSolution[3], (float)OptimalSolution[4], (float)OptimalSolution[5]),
+        make_float3((float)OptimalSolution[0], (float)OptimalSolution[1], (float)OptimalSolution[2]),
+        make_float3((float)OptimalSolution[6], (float)OptimalSolution[7], (float)OptimalSolution[8]),
+        make_float3((float)OptimalSolution[9], (float)OptimalSolution[10], (float)OptimalSolution[11]),
+        out);
 }
 void Optimizers3D::GetOptimalTransform() { GetOptimalTransform(MovingToFixed_Optimal.data()); }
 
@@ -805,8 +806,9 @@ Optimizers2D::Optimizers2D(std::string FilePath) : OptimizerBase(6, 3) {
 
 void Optimizers2D::GetOptimalTransform(float* out) {
     static_cast<Interpolator2D*>(interp_)->GetMovingToFixedTransform(
-        (float)OptimalSolution[2], make_float2(OptimalSolution[0], OptimalSolution[1]),
-        make_float2(OptimalSolution[3], OptimalSolution[4]), (float)OptimalSolution[5], out);
+        (float)OptimalSolution[2], make_float2((float)OptimalSolution[0], (float)OptimalSolution[1]),
+        make_float2((float)OptimalSolution[3], (float)OptimalSolution[4]), (float)OptimalSolution[5],
+        out);
 }
 void Optimizers2D::GetOptimalTransform() { GetOptimalTransform(MovingToFixed_Optimal.data()); }
 
@@ -814,9 +816,10 @@ Optimizers2DTo3D::Optimizers2DTo3D(std::string FilePath) : OptimizerBase(12, 4) 
 
 void Optimizers2DTo3D::GetOptimalTransform(float* out) {
     static_cast<Interpolator2DTo3D*>(interp_)->GetMovingToFixedTransform(
-        make_float3(OptimalSolution[3], OptimalSolution[4], OptimalSolution[5]),
-        make_float3(OptimalSolution[0], OptimalSolution[1], OptimalSolution[2]),
-        make_float3(OptimalSolution[6], OptimalSolution[7], OptimalSolution[8]),
-        make_float3(OptimalSolution[9], OptimalSolution[10], OptimalSolution[11]), out);
+        make_float3((float)OptimalSolution[3], (float)OptimalSolution[4], (float)OptimalSolution[5]),
+        make_float3((float)OptimalSolution[0], (float)OptimalSolution[1], (float)OptimalSolution[2]),
+        make_float3((float)OptimalSolution[6], (float)OptimalSolution[7], (float)OptimalSolution[8]),
+        make_float3((float)OptimalSolution[9], (float)OptimalSolution[10], (float)OptimalSolution[11]),
+        out);
 }
 void Optimizers2DTo3D::GetOptimalTransform() { GetOptimalTransform(MovingToFixed_Optimal.data()); }

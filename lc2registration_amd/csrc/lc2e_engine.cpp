@@ -860,8 +860,8 @@ int lc2e_pair_autotune(lc2e_pair* p, int n, const float* mats, int iters, int* b
         HIP_TRY(hipEventSynchronize(e1));
         float ms = 0.0f;
         HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-        if (ms / iters < bms) {
-            bms = ms / iters;
+        if (ms / (double)iters < bms) {
+            bms = ms / (double)iters;
             best = zc;
         }
     }
