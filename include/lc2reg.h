@@ -39,6 +39,15 @@ int lc2reg_minimize_qmtr(int d, const double* x0, const double* lb, const double
                          double xtol_rel, int maxeval, lc2reg_batch_fn f, void* user,
                          double* x_out, double* f_out, int* nevals);
 
+/* The reference's LC2 classes in one call: LC2_3D_class(x, y, z, US, HR) (z > 1) or
+ * LC2_2D_class(x, y, US, HR) (z == 1) on two images that already share a grid
+ * (cu_lc2_3d.cuh:14-55, cu_lc2_2d.cuh:14-47), then GetSimilarityMetric(..., patch_size).
+ * us and hr hold x*y*z floats, x fastest. Optional outputs (NULL to skip), x*y*z floats
+ * each: GetHighResolutionGradientBuffer (|Sobel| of hr) and GetWeightedMapBuffer
+ * (measure / w per voxel, 0 where w = 0). */
+int lc2reg_lc2_class_metric(int x, int y, int z, const float* us, const float* hr, int patch_size,
+                            double* out_lc2, float* out_gradient, float* out_weighted_map);
+
 /* Parse a parameters*.yaml with the configuration reader and write it back out. */
 int lc2reg_yaml_roundtrip(const char* in_path, const char* out_path);
 

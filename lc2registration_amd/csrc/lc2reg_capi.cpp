@@ -5,6 +5,7 @@
 #include <exception>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "lc2_registration.hpp"
 #include "optim.hpp"
@@ -50,6 +51,37 @@ lc2opt::BatchObjective wrap(lc2reg_batch_fn f, void* user) {
 extern "C" {
 
 const char* lc2reg_last_error(void) { return g_err.c_str(); }
+
+int lc2reg_lc2_class_metric(int x, int y, int z, const float* us, const float* hr, int patch_size,
+                            double* out_lc2, float* out_gradient, float* out_weighted_map) {
+    if (x <= 0 || y <= 0 || z <= 0 || !us || !hr || patch_size < 1) {
+        g_err = "lc2reg_lc2_class_metric: invalid argument";
+        return -1;
+    }
+    try {
+        const size_t n = (size_t)x * y * z;
+        std::vector<float> U(us, us + n), H(hr, hr + n);
+        std::unique_ptr<LC2_3D_class> c;
+        if (z == 1) c = std::make_unique<LC2_2D_class>(x, y, U, H);
+        else c = std::make_unique<LC2_3D_class>(x, y, z, U, H);
+        c->PrepareGradientFilter();
+        c->CalculateGradient();
+        const double l = z == 1 ? static_cast<LC2_2D_class*>(c.get())->GetSimilarityMetric(16, 16, patch_size)
+                                : c->GetSimilarityMetric(8, 8, 8, patch_size);
+        if (out_lc2) *out_lc2 = l;
+        if (out_gradient) {
+            const std::vector<float> g = c->GetHighResolutionGradientBuffer();
+            std::memcpy(out_gradient, g.data(), n * sizeof(float));
+        }
+        if (out_weighted_map) {
+            const std::vector<float> w = c->GetWeightedMapBuffer();
+            std::memcpy(out_weighted_map, w.data(), n * sizeof(float));
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
 
 int lc2reg_register_yaml(const char* mode, const char* yaml_path, int write_images,
                          double* out_params, double* out_lc2, int* out_nevals, float* out_matrix) {
