@@ -778,13 +778,17 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
             if (s >= nsteps) return;  // block-uniform
             const int q = s - LAG;
             const int zp = z0 - R + q;
-            // 1. finish the plane issued last step, start the next one
+            // 1. finish the plane issued last step, start the next one. The finishing
+            // wave runs at raised priority so that every wave reaches the first barrier
+            // sooner (+0.3%, measured).
+            __builtin_amdgcn_s_setprio(2);
 #pragma unroll
             for (int j = 0; j < SPT; ++j) {
                 const int i = tid + j * NTHR;
                 const float v = finish3(pend[j]);
                 if (i < NI) s_in[i] = v;
             }
+            __builtin_amdgcn_s_setprio(0);
             float cur0[PPT], cur1[PPT];
 #pragma unroll
             for (int j = 0; j < PPT; ++j) {
@@ -1024,7 +1028,8 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
     issue_static(f0);
     issue_stats(f0);
     for (int f = f0; f < f1; ++f) {
-        // 1. the warped tile of frame f
+        // 1. the warped tile of frame f (raised wave priority, as in the 3D kernel)
+        __builtin_amdgcn_s_setprio(2);
 #pragma unroll
         for (int j = 0; j < SPT; ++j) {
             if (j * kThreads >= NI) break;  // uniform
@@ -1034,6 +1039,7 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
             else v = finish2(pend[j]);
             if (i < NI) s_in[i] = v;
         }
+        __builtin_amdgcn_s_setprio(0);
         float cur0[PPT], cur1[PPT];
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
