@@ -866,6 +866,8 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
 #pragma unroll
             for (int c = 0; c < NC; ++c) ring[slot][c] = s_prod[c * TX * TY + tid];
             if (q >= 2 * R) {
+                // the FP64 ring sums and combine also run at raised priority (+0.4%, measured)
+                __builtin_amdgcn_s_setprio(2);
                 const double mu = mu_pre, cyy = cyy_pre;
                 if (q + 1 < nout + 2 * R) issue_stats(zp - R + 1);
                 double S[NC];
@@ -891,6 +893,7 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
                         }
                     }
                 }
+                __builtin_amdgcn_s_setprio(0);
                 // Partial sums per block of kZBlock output planes and tile column, at
                 // fixed positions in z: every z-chunk (a multiple of kZBlock) and every
                 // batch size then adds the same numbers in the same order.
