@@ -590,7 +590,10 @@ __device__ __forceinline__ void window_run(const V* in, int stride, V* out, int 
 // Rows (x pass) are cut into runs of 16 outputs and columns (y pass) into runs of 8, one
 // task each (runs of 8 put twice as many threads to work in the y pass); neighbouring
 // lanes take neighbouring rows (x pass) or columns (y pass).
-template <int NC, bool CNT, int TX, int TY, int NTHR, int W, int XS>
+// PRIO_X / PRIO_Y: run the x / y pass at raised wave priority. The 2D kernel gains 3.2%
+// from both; the 3D kernel loses 0.3% from either (measured), so it keeps the defaults.
+template <int NC, bool CNT, int TX, int TY, int NTHR, int W, int XS, bool PRIO_X = false,
+          bool PRIO_Y = false>
 __device__ __forceinline__ void box_xy(const double* s_prod, const int* s_cnt, double* s_xp,
                                        int* s_xcnt, double* s_out, int* s_ocnt, int ps, int py,
                                        int tid) {
@@ -599,6 +602,7 @@ __device__ __forceinline__ void box_xy(const double* s_prod, const int* s_cnt, d
     constexpr int NT = NC + (CNT ? 1 : 0);
     constexpr int SX = TX / 16;
     const int np = ps * py;
+    if constexpr (PRIO_X) __builtin_amdgcn_s_setprio(2);
     for (int t = tid; t < NT * py * SX; t += NTHR) {
         const int row = t % py, rest = t / py;
         const int seg = rest % SX, c = rest / SX;
@@ -608,8 +612,10 @@ __device__ __forceinline__ void box_xy(const double* s_prod, const int* s_cnt, d
         else
             window_run<W>(s_cnt + row * ps + seg * 16, 1, s_xcnt + row * XS + seg * 16, 1);
     }
+    if constexpr (PRIO_X) __builtin_amdgcn_s_setprio(0);
     __syncthreads();
     constexpr int SY8 = TY / 8;  // y runs of 8 outputs
+    if constexpr (PRIO_Y) __builtin_amdgcn_s_setprio(2);
     for (int t = tid; t < NT * TX * SY8; t += NTHR) {
         const int col = t % TX, rest = t / TX;
         const int seg = rest % SY8, c = rest / SY8;
@@ -619,6 +625,7 @@ __device__ __forceinline__ void box_xy(const double* s_prod, const int* s_cnt, d
         else
             window_run<W, int, 8>(s_xcnt + seg * 8 * XS + col, XS, s_ocnt + seg * 8 * TX + col, TX);
     }
+    if constexpr (PRIO_Y) __builtin_amdgcn_s_setprio(0);
     __syncthreads();
 }
 
@@ -630,7 +637,7 @@ __device__ __forceinline__ void box_xy_r(int r, const double* s_prod, const int*
                                          double* s_xp, int* s_xcnt, double* s_out, int* s_ocnt,
                                          int tid) {
     if (r == RR)
-        box_xy<NC, CNT, TX, TY, NTHR, 2 * RR + 1, lds_stride_2d(TX + 1)>(
+        box_xy<NC, CNT, TX, TY, NTHR, 2 * RR + 1, lds_stride_2d(TX + 1), true, true>(
             s_prod, s_cnt, s_xp, s_xcnt, s_out, s_ocnt, lds_stride_2d(TX + 2 * RR), TY + 2 * RR,
             tid);
     else if constexpr (RR < RMAX)
