@@ -161,17 +161,33 @@ struct Tiling {
 
 int round_zchunk(int c) { return (c + lc2e::kZBlock - 1) / lc2e::kZBlock * lc2e::kZBlock; }
 
-// z-chunk for n poses when nothing is tuned: the 3D kernel runs one 512-thread work-group
-// per CU, every work-group marches chunk + 2R + 2 planes, so a launch costs about
-// ceil(work-groups / CUs) rounds x (chunk + 2R + 2) plane steps. Minimise that over the
-// chunk lengths (multiples of kZBlock; ties: fewer, longer chunks).
-int model_zchunk(const lc2e_pair* p, int n) {
+lc2e::FusedArgs make_args(lc2e_pair* p, const float* d_mats, double* d_partials, const Tiling& t);
+
+// Fused work-groups one CU holds at once for this pair's kernel (queried once, then kept).
+int resident_per_cu(lc2e_pair* p) {
+    if (p->resident_per_cu <= 0) {
+        const lc2e::FusedArgs a = make_args(p, nullptr, nullptr, Tiling{});
+        int per_cu = 0;
+        if (lc2e::fused_resident_per_cu(a, p->info.grid_ndim, p->info.src_ndim,
+                                        p->info.high_res_is_moving, &per_cu) != hipSuccess ||
+            per_cu <= 0)
+            per_cu = 1;
+        p->resident_per_cu = per_cu;
+    }
+    return p->resident_per_cu;
+}
+
+// z-chunk for n poses when nothing is tuned: every work-group marches chunk + 2R + 2
+// planes, and the CUs hold resident_per_cu work-groups each at once, so a launch costs
+// about ceil(work-groups / (CUs x resident)) rounds x (chunk + 2R + 2) plane steps.
+// Minimise that over the chunk lengths (multiples of kZBlock; ties: fewer, longer chunks).
+int model_zchunk(lc2e_pair* p, int n) {
     const int r = p->info.patch_radius;
     const int ez = p->v.Z - 2 * r;
     int ntx, nty, nzc;
     if (ez <= 0 || lc2e::fused_blocks_3d(p->v, r, ez, &ntx, &nty, &nzc) <= 0) return lc2e::kZBlock;
     const long long cols = (long long)ntx * nty;
-    const int cus = p->e->cus > 0 ? p->e->cus : 256;
+    const int cus = (p->e->cus > 0 ? p->e->cus : 256) * resident_per_cu(p);
     const int cmax = round_zchunk(ez);
     int best = cmax;
     long long best_cost = -1;
@@ -379,17 +395,10 @@ int engine_device(lc2e_engine* e) { return e->device; }
 double launch_share(lc2e_pair* p, int n) {
     const Tiling t = tiling_for(p, n);
     if (t.tiles <= 0 || n <= 0) return 0.0;
-    if (p->resident_per_cu <= 0) {
-        const FusedArgs a = make_args(p, nullptr, nullptr, t);
-        int per_cu = 0;
-        if (fused_resident_per_cu(a, p->info.grid_ndim, p->info.src_ndim,
-                                  p->info.high_res_is_moving, &per_cu) != hipSuccess || per_cu <= 0)
-            per_cu = 1;
-        p->resident_per_cu = per_cu;
-    }
+    const int per_cu = resident_per_cu(p);
     const double wgs = (double)((t.tiles + kXcds - 1) / kXcds * kXcds) * n;
     const int cus = p->e->cus > 0 ? p->e->cus : 256;
-    return wgs / ((double)cus * p->resident_per_cu);
+    return wgs / ((double)cus * per_cu);
 }
 }  // namespace lc2e
 

@@ -11,8 +11,8 @@ namespace lc2e {
 constexpr int kTile2X = 16;    // output tile of the 2D fused kernel: kTile2X x kTile2Y,
 constexpr int kTile2Y = 16;    // one thread per output
 constexpr int kThreads = kTile2X * kTile2Y;  // work-group size of the 2D fused kernel
-constexpr int kTile3X = 32;    // output tile of the 3D fused kernel: 32 x 16 columns,
-constexpr int kTile3Y = 16;    // one work-group of 512 threads, one column per thread
+constexpr int kTile3X = 16;    // output tile of the 3D fused kernel: 16 x 16 columns,
+constexpr int kTile3Y = 16;    // one work-group of 256 threads (two per CU), one column each
 // Resident volumes carry this many zeroed bytes past their last voxel (paired x loads).
 constexpr size_t kVolumePad = 64;
 constexpr int kMaxR3 = 4;      // largest patch radius of the 3D fused kernel
