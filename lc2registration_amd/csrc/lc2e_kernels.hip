@@ -591,7 +591,7 @@ __device__ __forceinline__ void window_run(const V* in, int stride, V* out, int 
 // task each (runs of 8 put twice as many threads to work in the y pass); neighbouring
 // lanes take neighbouring rows (x pass) or columns (y pass).
 // PRIO_X / PRIO_Y: run the x / y pass at raised wave priority. The 2D kernel gains 3.2%
-// from both; the 3D kernel loses 0.3% from either (measured), so it keeps the defaults.
+// from both; the 3D kernel gains nothing (measured), so it keeps the defaults.
 template <int NC, bool CNT, int TX, int TY, int NTHR, int W, int XS, bool PRIO_X = false,
           bool PRIO_Y = false>
 __device__ __forceinline__ void box_xy(const double* s_prod, const int* s_cnt, double* s_xp,
@@ -873,8 +873,6 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
 #pragma unroll
             for (int c = 0; c < NC; ++c) ring[slot][c] = s_prod[c * TX * TY + tid];
             if (q >= 2 * R) {
-                // the FP64 ring sums and combine also run at raised priority (+0.4%, measured)
-                __builtin_amdgcn_s_setprio(2);
                 const double mu = mu_pre, cyy = cyy_pre;
                 if (q + 1 < nout + 2 * R) issue_stats(zp - R + 1);
                 double S[NC];
@@ -900,7 +898,6 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
                         }
                     }
                 }
-                __builtin_amdgcn_s_setprio(0);
                 // Partial sums per block of kZBlock output planes and tile column, at
                 // fixed positions in z: every z-chunk (a multiple of kZBlock) and every
                 // batch size then adds the same numbers in the same order.
