@@ -590,8 +590,9 @@ __device__ __forceinline__ void window_run(const V* in, int stride, V* out, int 
 // Rows (x pass) are cut into runs of 16 outputs and columns (y pass) into runs of 8, one
 // task each (runs of 8 put twice as many threads to work in the y pass); neighbouring
 // lanes take neighbouring rows (x pass) or columns (y pass).
-// PRIO_X / PRIO_Y: run the x / y pass at raised wave priority. The 2D kernel gains 3.2%
-// from both; the 3D kernel gains nothing (measured), so it keeps the defaults.
+// PRIO_X / PRIO_Y: run the x / y pass at raised wave priority. Both fused kernels set them:
+// with several work-groups per CU out of phase, the LDS passes gain from going first
+// (3D +1.3%, 2D +3.2%, measured).
 template <int NC, bool CNT, int TX, int TY, int NTHR, int W, int XS, bool PRIO_X = false,
           bool PRIO_Y = false>
 __device__ __forceinline__ void box_xy(const double* s_prod, const int* s_cnt, double* s_xp,
@@ -859,7 +860,7 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
             if (s + 1 < nsteps) issue_plane(zw0 + s + 1);
             if (q < 0) return;  // block-uniform: Sobel ring warm-up only
             // 3. in-plane box sums
-            box_xy<NC, CNT, TX, TY, NTHR, W, XS>(s_prod, s_cnt, s_xp, s_xcnt, s_prod, s_cnt, PS,
+            box_xy<NC, CNT, TX, TY, NTHR, W, XS, true, true>(s_prod, s_cnt, s_xp, s_xcnt, s_prod, s_cnt, PS,
                                                  PY, tid);
             // 4. z ring of plane sums and combine. The patch sums add the W plane sums in
             // z order (no running sum), so a voxel's value does not depend on where its
