@@ -219,8 +219,8 @@ public:
     void GenerateMappers(float3 RotationAngle, float3 Translation);
     void GenerateMappers(float3 RotationAngle, float3 Translation, float3 scale, float3 shear);
     void GetCurrentTransform(float _t[16]);
-    void Interpolate(const int Blk_Dim_x = 8, const int Blk_Dim_y = 8, const int Blk_Dim_z = 1);
-    double GetSimilarityMeasure(const int patchSize = 3);
+    void Interpolate(const int Blk_Dim_x = 8, const int Blk_Dim_y = 8, const int Blk_Dim_z = 8);
+    double GetSimilarityMeasure(const int patchSize = 9);
     void GetMovingToFixedTransform(float3 RotationAngle, float3 Translation, float* outMovingToFixed);
     void GetMovingToFixedTransform(float3 RotationAngle, float3 Translation, float3 scale,
                                    float3 shear, float* outMovingToFixed);

@@ -167,7 +167,8 @@ int lc2e_pair_set_tuning_n(lc2e_pair* p, int n, int zchunk);
 int lc2e_pair_get_tuning(lc2e_pair* p, int* zchunk);
 /* Autotuner: times the fused kernel over candidate z-chunks (HIP events, `iters` launches
  * each after one warm-up) for batches of n poses `mats`, records the fastest for batches
- * of n (clearing a forced value) and returns it. 2D pairs have nothing to tune and
+ * of n and returns it. A z-chunk forced with lc2e_pair_set_tuning stays in force (and
+ * takes precedence) until it is cleared with 0. 2D pairs have nothing to tune and
  * return 1. No counterpart in the reference. */
 int lc2e_pair_autotune(lc2e_pair* p, int n, const float* mats, int iters, int* best_zchunk,
                        double* best_ms);

@@ -874,7 +874,7 @@ int lc2e_pair_autotune(lc2e_pair* p, int n, const float* mats, int iters, int* b
             best = zc;
         }
     }
-    p->zchunk_forced = 0;  // the tuned value now applies to batches of n
+    p->zchunk_forced = forced;  // a forced z-chunk keeps precedence over the tuned value
     p->tuned[n] = best;
     if (best_zchunk) *best_zchunk = best;
     if (best_ms) *best_ms = bms;

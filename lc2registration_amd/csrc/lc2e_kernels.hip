@@ -15,8 +15,8 @@
 // FP32 values are exact in FP64), which removes the cancellation an FP32
 // sliding-window sum would suffer on intensities with small local variance.
 //
-// 3D structure (k_fused3d): a 512-thread work-group owns a 32x16 column of
-// output voxels over `zchunk` planes and marches along z. Per input plane:
+// 3D structure (k_fused3d): a 256-thread work-group (two per CU) owns a 16x16 column
+// of output voxels over `zchunk` planes and marches along z. Per input plane:
 // warp the plane (+ Sobel halo) into LDS, separable Sobel of each 3x3 neighbourhood
 // with a 3-plane register ring along z, FP64 products into LDS,
 // sliding x then y box sums in LDS, and a (2R+1)-plane ring of plane sums in
@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "lc2e_kernels.hpp"
@@ -1514,14 +1515,16 @@ hipError_t launch3r(const FusedArgs& a, bool movi, dim3 grid, hipStream_t st, in
 template <bool MOVI, bool SRC3D, typename T, int RMAX>
 hipError_t launch2r(const FusedArgs& a, dim3 grid, hipStream_t st, int* occ) {
     const size_t lds = lds_bytes_2d(a.r, MOVI);
-    static bool configured = false;  // allow this variant's largest radius once
-    if (!configured) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused2d<MOVI, SRC3D, T, RMAX>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)lds_bytes_2d(RMAX, MOVI));
-        if (e != hipSuccess) return e;
-        configured = true;
-    }
+    // allow this variant's largest radius, once per process (client threads and a serving
+    // worker may get here at the same time)
+    static std::once_flag once;
+    static hipError_t set_err = hipSuccess;
+    std::call_once(once, [] {
+        set_err = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused2d<MOVI, SRC3D, T, RMAX>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds_bytes_2d(RMAX, MOVI));
+    });
+    if (set_err != hipSuccess) return set_err;
     if (occ) return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_fused2d<MOVI, SRC3D, T, RMAX>,
                                                                   kThreads, lds);
     hipLaunchKernelGGL((k_fused2d<MOVI, SRC3D, T, RMAX>), grid, dim3(kThreads), lds, st, a);
@@ -1678,16 +1681,17 @@ hipError_t launch_mi(const FusedArgs& a, int grid_ndim, int src_ndim, int movi, 
     const int chunk = 32768;  // voxels per work-group, below the 16-bit counter limit
     const dim3 gj((unsigned)((n + chunk - 1) / chunk), (unsigned)n_poses);
     const size_t lds = (size_t)((nb + 1) / 2) * sizeof(unsigned);
-    static bool configured = false;
-    if (!configured) {
+    static std::once_flag once;
+    static hipError_t set_err = hipSuccess;
+    std::call_once(once, [] {
         for (const void* f : {reinterpret_cast<const void*>(&k_mi_joint<float>),
                               reinterpret_cast<const void*>(&k_mi_joint<__half>)}) {
-            e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)((kMiMaxBins * kMiMaxBins / 2) * sizeof(unsigned)));
-            if (e != hipSuccess) return e;
+            set_err = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)((kMiMaxBins * kMiMaxBins / 2) * sizeof(unsigned)));
+            if (set_err != hipSuccess) return;
         }
-        configured = true;
-    }
+    });
+    if (set_err != hipSuccess) return set_err;
     const int stat_is_u = movi ? 1 : 0;
     if (a.src_f16)
         hipLaunchKernelGGL(k_mi_joint<__half>, gj, dim3(512), lds, st, a, g3, s3, stat, stat_is_u,
