@@ -7,11 +7,29 @@
 #ifndef LC2REG_H
 #define LC2REG_H
 
+#include "lc2e.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
 
 const char* lc2reg_last_error(void);
+
+/* Host-only geometry (no device needed), the same functions the engine and the facade run:
+ * lc2reg_pair_geometry fills info->virt, fixed_center, shift, grid_ndim and src_ndim (the
+ * other fields 0) for fixed/moving geometries as lc2e_pair_create does (virtual grid
+ * interpolator.cu:320, :337-339; centre and CenterOverlaid shift :495-529);
+ * lc2reg_pose_to_matrix is lc2e_pose_to_matrix on such an info (spatialMappers.cuh:579-850,
+ * :977-1033); lc2reg_moving_to_fixed is GetMovingToFixedTransform (interpolator.cu:633-679,
+ * :1382-1425): inv(VirtualToMoving) as 16 floats (3D sources) or the 2D 3x3 (9 floats),
+ * row-major, bottom row forced. */
+int lc2reg_pair_geometry(const lc2e_geom* fixed, const lc2e_geom* moving,
+                         const float resampled_spacing[3], int dim, int center_overlaid,
+                         lc2e_pair_info* out);
+int lc2reg_pose_to_matrix(const lc2e_pair_info* info, const double* params, int n_params,
+                          float out12[12]);
+int lc2reg_moving_to_fixed(const lc2e_pair_info* info, const double* params, int n_params,
+                           float* out);
 
 /* One registration as the reference's CLI runs it for -m 2D | 3D | 2DTo3D -c <yaml>
  * (main.cu:728-827): Interpolator + Optimizers from the YAML, DIRECT stage and BOBYQA

@@ -226,14 +226,7 @@ void InterpolatorBase::moving_to_fixed(const double* params, int n, float* out, 
     if (!pair_) InitiateRegistrationHandle();
     float m[12];
     check(lc2e_pose_to_matrix(pair_, params, n, m), "pose");
-    float inv[16];
-    if (!lc2e::invert_affine(m, inv)) throw std::runtime_error("singular transform");
-    if (out_dim == 4) {
-        for (int i = 0; i < 16; ++i) out[i] = inv[i];
-    } else {
-        const float o[9] = {inv[0], inv[1], inv[3], inv[4], inv[5], inv[7], 0, 0, 1};
-        for (int i = 0; i < 9; ++i) out[i] = o[i];
-    }
+    if (!lc2e::moving_to_fixed(m, out_dim, out)) throw std::runtime_error("singular transform");
 }
 
 // WriteOut (interpolator.cu:681-920): both images on the virtual grid as NRRD, and the

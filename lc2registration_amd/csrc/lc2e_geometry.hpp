@@ -147,4 +147,19 @@ inline bool invert_affine(const float m12[12], float out16[16]) {
     return true;
 }
 
+// GetMovingToFixedTransform (interpolator.cu:633-679 3D, :1382-1425 2D, :1729-1775 2D->3D):
+// VirtualToFixed (the identity) times inv(VirtualToMoving), bottom row forced to [0,0,0,1];
+// out is 4x4 row-major (out_dim 4) or the 2D 3x3 (out_dim 3). Returns false if singular.
+inline bool moving_to_fixed(const float m12[12], int out_dim, float* out) {
+    float inv[16];
+    if (!invert_affine(m12, inv)) return false;
+    if (out_dim == 4) {
+        for (int i = 0; i < 16; ++i) out[i] = inv[i];
+    } else {
+        const float o[9] = {inv[0], inv[1], inv[3], inv[4], inv[5], inv[7], 0, 0, 1};
+        for (int i = 0; i < 9; ++i) out[i] = o[i];
+    }
+    return true;
+}
+
 }  // namespace lc2e

@@ -405,23 +405,26 @@ __device__ __forceinline__ double rsq64(double x) {
     return r;
 }
 
-// s^T C^+ s for the two-column centred fit with the rank rule shared with the oracle.
+// s^T C^+ s for the two-column centred fit. Rank rule: a column whose centred energy is
+// at most kTauConst of its raw energy is dropped; two columns with det <= kTauColl*a*b
+// are collinear and U is projected on the leading singular direction of the centred,
+// unit-normalised columns, I' + sign(rho) G' (of squared norm 2 + 2|rho|): least squares
+// by truncated SVD with the cutoff sigma_min/sigma_max <= sqrt(kTauColl)/2.
 __device__ __forceinline__ double explained_var(double a, double b, double c, double p, double q,
                                                 double rawa, double rawb) {
-    const bool iconst = !(a > kTauConst * rawa);
-    const bool gconst = !(b > kTauConst * rawb);
-    if (iconst) { a = 0.0; c = 0.0; p = 0.0; }
-    if (gconst) { b = 0.0; c = 0.0; q = 0.0; }
-    const double det = a * b - c * c;
-    if (a > 0.0 && b > 0.0 && det > kTauColl * a * b)
-        return (b * p * p - 2.0 * c * p * q + a * q * q) * rcp64(det);
-    const double tr = a + b;
-    if (tr > 0.0) {
-        // sqrt and division through the refined hardware estimates (a few ulp), as above
-        const double sa = a > 0.0 ? a * rsq64(a) : 0.0, sb = b > 0.0 ? b * rsq64(b) : 0.0;
-        const double u = sa * p + (c < 0.0 ? -1.0 : 1.0) * sb * q;
-        return (u * u) * rcp64(tr * tr);
+    const bool ikeep = a > kTauConst * rawa;
+    const bool gkeep = b > kTauConst * rawb;
+    if (ikeep && gkeep) {
+        const double det = a * b - c * c;
+        if (det > kTauColl * a * b)
+            return (b * p * p - 2.0 * c * p * q + a * q * q) * rcp64(det);
+        // reciprocal square roots through the refined hardware estimates (a few ulp)
+        const double ra = rsq64(a), rb = rsq64(b);
+        const double t = p * ra + (c < 0.0 ? -1.0 : 1.0) * q * rb;
+        return (t * t) * rcp64(2.0 + 2.0 * fabs(c) * ra * rb);
     }
+    if (ikeep) return p * p * rcp64(a);
+    if (gkeep) return q * q * rcp64(b);
     return 0.0;
 }
 

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "lc2_registration.hpp"
+#include "lc2e_geometry.hpp"
 #include "optim.hpp"
 #include "yaml_lite.hpp"
 
@@ -51,6 +52,48 @@ lc2opt::BatchObjective wrap(lc2reg_batch_fn f, void* user) {
 extern "C" {
 
 const char* lc2reg_last_error(void) { return g_err.c_str(); }
+
+int lc2reg_pair_geometry(const lc2e_geom* fixed, const lc2e_geom* moving,
+                         const float resampled_spacing[3], int dim, int center_overlaid,
+                         lc2e_pair_info* out) {
+    if (!fixed || !moving || !resampled_spacing || !out || (dim != 2 && dim != 3)) {
+        g_err = "lc2reg_pair_geometry: invalid argument";
+        return -1;
+    }
+    const int src_ndim = moving->dims[2] > 1 ? 3 : 2;
+    const lc2e::PairGeometry pg = lc2e::make_pair_geometry(*fixed, *moving, resampled_spacing, dim,
+                                                           src_ndim, center_overlaid != 0);
+    *out = lc2e_pair_info{};
+    out->virt = pg.virt;
+    for (int a = 0; a < 3; ++a) {
+        out->fixed_center[a] = pg.fixed_center[a];
+        out->shift[a] = pg.shift[a];
+    }
+    out->grid_ndim = dim;
+    out->src_ndim = src_ndim;
+    out->n_frames = dim == 2 ? pg.virt.dims[2] : 1;
+    return 0;
+}
+
+int lc2reg_pose_to_matrix(const lc2e_pair_info* info, const double* params, int n_params,
+                          float out12[12]) {
+    if (!info || !params || !out12 || lc2e::pose_to_matrix(*info, params, n_params, out12) != 0) {
+        g_err = "lc2reg_pose_to_matrix: invalid argument";
+        return -1;
+    }
+    return 0;
+}
+
+int lc2reg_moving_to_fixed(const lc2e_pair_info* info, const double* params, int n_params,
+                           float* out) {
+    float m[12];
+    if (lc2reg_pose_to_matrix(info, params, n_params, m) != 0) return -1;
+    if (!lc2e::moving_to_fixed(m, info->src_ndim == 3 ? 4 : 3, out)) {
+        g_err = "lc2reg_moving_to_fixed: singular transform";
+        return -1;
+    }
+    return 0;
+}
 
 int lc2reg_lc2_class_metric(int x, int y, int z, const float* us, const float* hr, int patch_size,
                             double* out_lc2, float* out_gradient, float* out_weighted_map) {

@@ -18,7 +18,10 @@
 /* Rank handling of the exact least-squares projection (the reference's CGLS
  * converges to this projection; it has no explicit rule of its own). A column
  * whose centred energy is below TAU_CONST of its raw energy is constant within
- * rounding; a pair whose Gram determinant is below TAU_COLL*a*b is collinear. */
+ * rounding and dropped; a pair whose Gram determinant is below TAU_COLL*a*b is
+ * collinear and U is projected on the leading singular direction of the two centred,
+ * unit-normalised columns. That is least squares by truncated SVD on those columns
+ * with the cutoff sigma_min/sigma_max <= sqrt(TAU_COLL)/2 (numpy lstsq, rcond 5e-6). */
 #define LC2O_TAU_CONST 1e-10
 #define LC2O_TAU_COLL 1e-10
 
@@ -91,7 +94,7 @@ void lc2o_affine_init_3d(const float rot_deg[3], const float trans[3], const flo
     mm3(t2, Sh, t3);
     mm3(t3, S, A);
     for (int i = 0; i < 3; ++i) {
-        double t = center[i] + trans[i];
+        double t = (double)center[i] + (double)trans[i];
         for (int k = 0; k < 3; ++k) t += A[i * 3 + k] * ((double)shift[k] - (double)center[k]);
         for (int k = 0; k < 3; ++k) out16[i * 4 + k] = (float)A[i * 3 + k];
         out16[i * 4 + 3] = (float)t;
@@ -112,7 +115,7 @@ void lc2o_affine_init_2d(float rot_deg, const float trans[2], const float scale[
     A[2] = scale[0] * s;
     A[3] = scale[1] * (s * shear + c);
     for (int i = 0; i < 2; ++i) {
-        double t = center[i] + trans[i];
+        double t = (double)center[i] + (double)trans[i];
         for (int k = 0; k < 2; ++k) t += A[i * 2 + k] * ((double)shift[k] - (double)center[k]);
         out9[i * 3 + 0] = (float)A[i * 2 + 0];
         out9[i * 3 + 1] = (float)A[i * 2 + 1];
@@ -398,18 +401,20 @@ void lc2o_sobel_2d_sep(const float* I, const int dims[2], float* G) {
  * rawa / rawb are the uncentred energies used as the scale of "zero". */
 static double explained_var(double a, double b, double c, double p, double q, double rawa,
                             double rawb) {
-    int iconst = !(a > LC2O_TAU_CONST * rawa);
-    int gconst = !(b > LC2O_TAU_CONST * rawb);
-    if (iconst) { a = 0.0; c = 0.0; p = 0.0; }
-    if (gconst) { b = 0.0; c = 0.0; q = 0.0; }
-    double det = a * b - c * c;
-    if (a > 0.0 && b > 0.0 && det > LC2O_TAU_COLL * a * b)
-        return (b * p * p - 2.0 * c * p * q + a * q * q) / det;
-    double tr = a + b;
-    if (tr > 0.0) {
-        double u = sqrt(a) * p + (c < 0.0 ? -1.0 : 1.0) * sqrt(b) * q;
-        return (u * u) / (tr * tr);
+    int ikeep = a > LC2O_TAU_CONST * rawa;
+    int gkeep = b > LC2O_TAU_CONST * rawb;
+    if (ikeep && gkeep) {
+        double det = a * b - c * c;
+        if (det > LC2O_TAU_COLL * a * b)
+            return (b * p * p - 2.0 * c * p * q + a * q * q) / det;
+        /* collinear: unit columns I', G' with correlation rho = c / sqrt(ab); the leading
+         * singular direction is I' + sign(rho) G', of squared norm 2 + 2|rho| */
+        double ra = 1.0 / sqrt(a), rb = 1.0 / sqrt(b);
+        double t = p * ra + (c < 0.0 ? -1.0 : 1.0) * q * rb;
+        return (t * t) / (2.0 + 2.0 * fabs(c) * ra * rb);
     }
+    if (ikeep) return p * p / a;
+    if (gkeep) return q * q / b;
     return 0.0;
 }
 

@@ -101,6 +101,12 @@ def test_yaml_registration_3d_recovers_known_pose(reg, tmp_path):
     assert o["IsRegistered"] is True and o["NumEvaluations"] == nev
     assert abs(o["LastSimilarity"] - lc2) < 1e-12
     assert np.allclose(np.array(o["TransformMatrix"]).ravel(), mat, rtol=1e-6, atol=1e-6)
+    # the reported MovingToFixed is the oracle's inverse of the oracle's affine matrix of the
+    # reported optimum (float parameters, fixed-image centre, no CenterOverlaid shift)
+    xf = x.astype(np.float32)
+    om = O.affine_3d(xf[:3], xf[3:6], xf[6:9], xf[9:12],
+                     O.center(O.geom(f.dims, f.spacing, f.origin), 3), np.zeros(3, np.float32))
+    assert np.array_equal(np.asarray(mat, np.float32).reshape(4, 4), O.inverse(om))
     opt = out["InputParameters"]["OptimizerSettings"]
     assert np.allclose([opt["InitialValues"][k] for k in KEYS_3D], x.astype(np.float32), atol=1e-6)
     assert opt["BOBYQA"][0]["Enabled"] is False and opt["BOBYQA"][1]["Enabled"] is True
