@@ -19,8 +19,9 @@
 //  * patchSize 0 selects the normalized mutual information (cu_lc2_3d.cu:1196-1301,
 //    256 x 256 bins) for 3D grids, as the reference does; the optimizers then maximise it.
 //    2D grids reject patchSize 0 (the reference runs LC2 with an empty patch there).
-//  * The optimizers' NLopt calls are served by lc2opt::direct_l / lc2opt::qmtr
-//    (optim.hpp) and every stage evaluates its candidate points in batches.
+//  * The optimizers' NLopt calls are served by lc2opt::opt (optim.hpp): DIRECT-L (both
+//    NLopt variants) and Powell's BOBYQA; DIRECT iterations and BOBYQA's initial
+//    interpolation set are evaluated in batches (one fused launch each).
 //  * Images come from raw files or NRRD (the reference reads through ITK).
 #pragma once
 
@@ -40,6 +41,12 @@ struct regFloatingPoints3D { float x, y, z; };
 struct regIntPoints2D { int x, y; };
 struct regFloatingPoints2D { float x, y; };
 
+// The reference's programmatic configuration (RegistrationParametersConfig.h:30-431): the
+// same members with the same defaults. The image fields feed the Interpolator* constructors
+// (interpolator.cu:393-492, :1105-1196); the optimizer fields feed Optimizers*(const
+// LC2Configuration*) -- declared by the reference (optimizers.cuh:102, :178) but never
+// defined there; here they map onto the YAML's OptimizerSettings as lc2reg::settings_from
+// documents.
 struct LC2Configuration2D {
     regIntPoints2D FixedDimension{0, 0};
     regFloatingPoints2D FixedSpacing{1.0f, 1.0f};
@@ -56,6 +63,21 @@ struct LC2Configuration2D {
     std::string ExportPath;
     bool IsCenterOverlaid = true;
     bool IsAffine = false;
+    // optimizers
+    float initFRE = 50.0f;
+    regFloatingPoints2D initTranslation{0.0f, 0.0f};
+    float initRotation = 0.0f;
+    regFloatingPoints2D initScale{1.0f, 1.0f};
+    float initShear = 0.0f;
+    regFloatingPoints2D boundTranslation_DIRECT{0.8f, 0.5f};
+    int MaxEval = 25;
+    regFloatingPoints2D boundTranslation_BOBYQA{1.0f, 1.0f};
+    float boundRotation_BOBYQA = 1.0f;
+    regFloatingPoints2D boundScale_BOBYQA{1.0f, 1.0f};
+    float boundShear_BOBYQA = 1.0f;
+    float RhoBegin = 0.1f;
+    float Tol = 1e-4f;
+    float MaxIterations = 1000;
     bool IsConfigured = false;
     float* MovingBuffer = nullptr;  // caller-owned float32 images (x fastest)
     float* FixedBuffer = nullptr;
@@ -64,7 +86,7 @@ struct LC2Configuration2D {
     float LastSimilarity = 0.0f;
     int NumberOfEvaluations = 0;
     std::vector<float> MovingToFixedMatrix = std::vector<float>{1, 0, 0, 0, 1, 0, 0, 0, 1};
-    void DumpRegistrtationResults(bool conv, float sim, int nev, const float* m) {
+    void DumpRegistrtationResults(const bool conv, const float sim, const int nev, const float* m) {
         IsConverged = conv;
         LastSimilarity = sim;
         NumberOfEvaluations = nev;
@@ -72,6 +94,7 @@ struct LC2Configuration2D {
     }
 };
 
+// The 3D and 2D->3D configurations have the same members and defaults.
 struct LC2Configuration3D {
     regIntPoints3D FixedDimension{0, 0, 0};
     regFloatingPoints3D FixedSpacing{1.0f, 1.0f, 1.0f};
@@ -81,33 +104,38 @@ struct LC2Configuration3D {
     regFloatingPoints3D MovingSpacing{1.0f, 1.0f, 1.0f};
     regFloatingPoints3D MovingOrigin{0.0f, 0.0f, 0.0f};
     std::string MovingImageFilePath;
-    bool IsFixedHighResolution = true;
+    bool IsFixedHighResolution = false;
     bool IsMovingHighResolution = false;
     regFloatingPoints3D SamplerSpacing{1.0f, 1.0f, 1.0f};
     int PatchSize = 3;
     std::string ExportPath;
     bool IsCenterOverlaid = true;
-    bool IsAffine = true;
+    bool IsAffine = false;
+    // optimizers
+    float initFRE = 30.0f;
+    regFloatingPoints3D initTranslation{0.0f, 0.0f, 0.0f};
+    regFloatingPoints3D initRotation{0.0f, 0.0f, 0.0f};
+    regFloatingPoints3D initScale{1.0f, 1.0f, 1.0f};
+    regFloatingPoints3D initShear{0.0f, 0.0f, 0.0f};
+    regFloatingPoints3D boundTranslation_DIRECT{0.5f, 0.5f, 0.5f};
+    int MaxEval = 100;
+    regFloatingPoints3D boundTranslation_BOBYQA{1.0f, 1.0f, 1.0f};
+    regFloatingPoints3D boundRotation_BOBYQA{1.0f, 1.0f, 1.0f};
+    regFloatingPoints3D boundScale_BOBYQA{1.0f, 1.0f, 1.0f};
+    regFloatingPoints3D boundShear_BOBYQA{1.0f, 1.0f, 1.0f};
+    float RhoBegin = 0.01f;
+    float Tol = 1e-6f;
+    float MaxIterations = 1000;
     bool IsConfigured = false;
     float* MovingBuffer = nullptr;
     float* FixedBuffer = nullptr;
-    // optimizer settings (the YAML's OptimizerSettings)
-    std::vector<double> InitialValues = {0, 0, 0, 0, 0, 0, 1, 1, 1, 0, 0, 0};
-    std::vector<double> LowerBounds = {-30, -30, -30, -15, -15, -15, 0.8, 0.8, 0.8, -0.2, -0.2, -0.2};
-    std::vector<double> UpperBounds = {30, 30, 30, 15, 15, 15, 1.2, 1.2, 1.2, 0.2, 0.2, 0.2};
-    bool DirectEnabled = true;
-    bool DirectEntries[4] = {true, false, false, false};  // Translation, Rotation, Scale, Shear
-    int DirectMaxEvaluations = 50;
-    std::vector<std::pair<bool, std::vector<bool>>> BobyqaRounds = {
-        {true, {true, true, false, false}}, {true, {true, true, true, true}}};
-    double BobyqaAbsTol = 1e-4;
     // results
     bool IsConverged = false;
     float LastSimilarity = 0.0f;
     int NumberOfEvaluations = 0;
     std::vector<float> MovingToFixedMatrix = std::vector<float>{1, 0, 0, 0, 0, 1, 0, 0,
                                                                 0, 0, 1, 0, 0, 0, 0, 1};
-    void DumpRegistrtationResults(bool conv, float sim, int nev, const float* m) {
+    void DumpRegistrtationResults(const bool conv, const float sim, const int nev, const float* m) {
         IsConverged = conv;
         LastSimilarity = sim;
         NumberOfEvaluations = nev;
@@ -115,7 +143,7 @@ struct LC2Configuration3D {
     }
 };
 
-using LC2Configuration2DTo3D = LC2Configuration3D;
+struct LC2Configuration2DTo3D : LC2Configuration3D {};
 
 namespace lc2yaml {
 class Node;
@@ -216,6 +244,7 @@ public:
 class Interpolator2DTo3D : public lc2reg::InterpolatorBase {
 public:
     Interpolator2DTo3D(const std::string yaml_filePath);
+    Interpolator2DTo3D(const LC2Configuration2DTo3D* ExternalConfig);
     void GenerateMappers(float3 RotationAngle, float3 Translation);
     void GenerateMappers(float3 RotationAngle, float3 Translation, float3 scale, float3 shear);
     void GetCurrentTransform(float _t[16]);
@@ -285,6 +314,8 @@ struct OptimizerSettings {
         bool enabled = false;
         std::vector<bool> entries = {true, true, false, false};
         double abs_tol = 1e-4;
+        int max_eval = 0;         // 0: no limit (the YAML rounds set none)
+        double rho_begin = 0.0;   // initial step as a fraction of each range (0: NLopt's default)
     };
     std::vector<Round> rounds;
     int patch_size = 3;
@@ -304,6 +335,10 @@ public:
     double GetLastSimilarity() const { return SimilarityMeasure; }
     // largest number of candidate poses per fused launch (default 64)
     void SetMaxBatch(int n) { max_batch_ = n > 0 ? n : 1; }
+    // method of the BOBYQA rounds: Powell's BOBYQA (default, the reference's LN_BOBYQA) or
+    // the batched diagonal-model trust region (more evaluations, fewer launches)
+    enum class LocalMethod { Bobyqa, Qmtr };
+    void SetLocalMethod(LocalMethod m) { local_method_ = m; }
 
 protected:
     OptimizerBase(int nparams, int mdim);
@@ -316,11 +351,15 @@ protected:
     void assign_back(const std::vector<double>& x);
     std::string result_path() const;
     void log_header();
-    void log_eval(const std::vector<double>& p, double sim);
+    void log_eval(const std::vector<double>& p, double sim, bool initial = false);
+    void apply_config(const std::vector<double>& initial, const std::vector<double>& halfwidth,
+                      const std::vector<double>& direct_halfwidth, int direct_maxeval,
+                      bool affine, double rho_begin, double tol, int max_iterations, int patch);
 
     InterpolatorBase* interp_ = nullptr;
     OptimizerSettings cfg_;
     std::vector<double> InitialValues, LowerBounds, UpperBounds, OptimalSolution;
+    std::vector<double> DirectLower, DirectUpper;  // DIRECT-stage bounds (configuration structs)
     std::vector<int> CurrentHolding_OptEntries;
     int nparams_, mdim_;
     int EvalCounter = 0;
@@ -331,8 +370,13 @@ protected:
     std::ofstream m_logFileId;
     int m_stepMarker = -1;
     bool direct_original_ = true;  // GN_ORIG_DIRECT_L (3D) vs GN_DIRECT_L (2D)
+    // RegistrationInfo.log: none (2D, as the reference), the 3D layout (optimizers.cu:610,
+    // :709-720) or the 2D->3D layout (:1263-1272, :1335-1340)
+    enum class LogLayout { None, Volume, Slice };
+    LogLayout log_layout_ = LogLayout::Volume;
     bool update_initial_on_export_ = true;
     int max_batch_ = 64;
+    LocalMethod local_method_ = LocalMethod::Bobyqa;
 };
 
 }  // namespace lc2reg
@@ -349,6 +393,7 @@ public:
 class Optimizers2D : public lc2reg::OptimizerBase {
 public:
     Optimizers2D(std::string FilePath);
+    Optimizers2D(const LC2Configuration2D* ExternalConfig);
     void LinkToInterpolator(Interpolator2D* external_Interpolator) { interp_ = external_Interpolator; }
     void GetOptimalTransform(float* outMatrix);
     void GetOptimalTransform();
@@ -357,6 +402,7 @@ public:
 class Optimizers2DTo3D : public lc2reg::OptimizerBase {
 public:
     Optimizers2DTo3D(std::string FilePath);
+    Optimizers2DTo3D(const LC2Configuration3D* ExternalConfig);
     void LinkToInterpolator(Interpolator2DTo3D* external_Interpolator) { interp_ = external_Interpolator; }
     void GetOptimalTransform(float* outMatrix);
     void GetOptimalTransform();

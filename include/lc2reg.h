@@ -57,6 +57,17 @@ int lc2reg_minimize_qmtr(int d, const double* x0, const double* lb, const double
                          double xtol_rel, int maxeval, lc2reg_batch_fn f, void* user,
                          double* x_out, double* f_out, int* nevals);
 
+/* The nlopt::opt-shaped optimizer facade (optim.hpp) that the Optimizers* classes use:
+ * algorithm 0 = GN_DIRECT_L, 1 = GN_ORIG_DIRECT_L, 2 = LN_BOBYQA (optimizers.cu:298, :854,
+ * :937). x0 is BOBYQA's start (ignored by DIRECT); step (NULL: NLopt's default initial
+ * step) its initial step per variable; xtol_rel its set_xtol_rel; maxeval the evaluation
+ * budget (required by DIRECT; <= 0: none for BOBYQA). Returns the NLopt result code
+ * (1 success, 4 xtol reached, 5 maxeval reached; negative on failure). */
+int lc2reg_minimize_nlopt(int algorithm, int d, const double* x0, const double* lb,
+                          const double* ub, const double* step, double xtol_rel, int maxeval,
+                          lc2reg_batch_fn f, void* user, double* x_out, double* f_out,
+                          int* nevals);
+
 /* The reference's LC2 classes in one call: LC2_3D_class(x, y, z, US, HR) (z > 1) or
  * LC2_2D_class(x, y, US, HR) (z == 1) on two images that already share a grid
  * (cu_lc2_3d.cuh:14-55, cu_lc2_2d.cuh:14-47), then GetSimilarityMetric(..., patch_size).

@@ -265,8 +265,10 @@ Interpolator3D::Interpolator3D(const LC2Configuration3D* c) {
                  {c->MovingSpacing.x, c->MovingSpacing.y, c->MovingSpacing.z},
                  {c->MovingOrigin.x, c->MovingOrigin.y, c->MovingOrigin.z}};
     const float rs[3] = {c->SamplerSpacing.x, c->SamplerSpacing.y, c->SamplerSpacing.z};
-    load_from_buffers(c->FixedBuffer, fg, c->MovingBuffer, mg, rs, c->IsMovingHighResolution,
-                      c->IsCenterOverlaid, c->PatchSize, 3, c->ExportPath);
+    // interpolator.cu:430-436: IsFixedHighResolution takes precedence
+    load_from_buffers(c->FixedBuffer, fg, c->MovingBuffer, mg, rs,
+                      !c->IsFixedHighResolution && c->IsMovingHighResolution, c->IsCenterOverlaid,
+                      c->PatchSize, 3, c->ExportPath);
     FixedFilePath = c->FixedImageFilePath;
     MovingFilePath = c->MovingImageFilePath;
 }
@@ -314,7 +316,7 @@ Interpolator2D::Interpolator2D(const LC2Configuration2D* c) {
                  {c->MovingSpacing.x, c->MovingSpacing.y, 1.0f},
                  {c->MovingOrigin.x, c->MovingOrigin.y, 0.0f}};
     const float rs[3] = {c->SamplerSpacing.x, c->SamplerSpacing.y, 1.0f};
-    load_from_buffers(c->FixedBuffer, fg, c->MovingBuffer, mg, rs, c->IsMovingHighResolution,
+    load_from_buffers(c->FixedBuffer, fg, c->MovingBuffer, mg, rs, !c->IsFixedHighResolution && c->IsMovingHighResolution,
                       c->IsCenterOverlaid, c->PatchSize, 2, c->ExportPath);
     FixedFilePath = c->FixedImageFilePath;
     MovingFilePath = c->MovingImageFilePath;
@@ -356,6 +358,24 @@ void Interpolator2D::WriteOut(float* m) { write_out(m, 3); }
 
 // ---- Interpolator2DTo3D ----
 Interpolator2DTo3D::Interpolator2DTo3D(const std::string yaml_filePath) { load_from_yaml(yaml_filePath, 2, true); }
+
+// Declared by the reference (interpolator.cuh:219) but never defined there: the fixed image
+// is the 2D slice (FixedDimension.z = 1), the moving one the 3D volume; 2D LC2 on the slice.
+Interpolator2DTo3D::Interpolator2DTo3D(const LC2Configuration2DTo3D* c) {
+    if (!c || !c->IsConfigured) throw std::runtime_error("LC2Configuration2DTo3D is not configured");
+    lc2e_geom fg{{c->FixedDimension.x, c->FixedDimension.y, std::max(1, c->FixedDimension.z)},
+                 {c->FixedSpacing.x, c->FixedSpacing.y, c->FixedSpacing.z},
+                 {c->FixedOrigin.x, c->FixedOrigin.y, c->FixedOrigin.z}};
+    lc2e_geom mg{{c->MovingDimension.x, c->MovingDimension.y, c->MovingDimension.z},
+                 {c->MovingSpacing.x, c->MovingSpacing.y, c->MovingSpacing.z},
+                 {c->MovingOrigin.x, c->MovingOrigin.y, c->MovingOrigin.z}};
+    const float rs[3] = {c->SamplerSpacing.x, c->SamplerSpacing.y, c->SamplerSpacing.z};
+    load_from_buffers(c->FixedBuffer, fg, c->MovingBuffer, mg, rs,
+                      !c->IsFixedHighResolution && c->IsMovingHighResolution, c->IsCenterOverlaid,
+                      c->PatchSize, 2, c->ExportPath);
+    FixedFilePath = c->FixedImageFilePath;
+    MovingFilePath = c->MovingImageFilePath;
+}
 
 void Interpolator2DTo3D::GenerateMappers(float3 R, float3 T) {
     const double p[6] = {T.x, T.y, T.z, R.x, R.y, R.z};
@@ -583,30 +603,36 @@ void OptimizerBase::assign_back(const std::vector<double>& x) {
         for (int j = 0; j < entry_width(e); ++j) OptimalSolution[entry_offset(e) + j] = x[k++];
 }
 
+// RegistrationInfo.log header: the 3D layout (optimizers.cu:610) or the 2D->3D layout with
+// its "Initial step" line (:1263-1264).
 void OptimizerBase::log_header() {
     if (!m_logFileId.is_open()) return;
-    m_logFileId << "Step,Transform[" << mdim_ * mdim_ << "],";
-    const char** keys = nparams_ == 12 ? kKeys3D : kKeys2D;
-    for (int i = 0; i < nparams_; ++i) m_logFileId << keys[i] << ",";
-    m_logFileId << "Similarity,-log(sim),IteratorCounter\n";
+    static const char* kCols =
+        "Translation_x,Translation_y,Translation_z,Rotation_x,Rotation_y,Rotation_z,Scale_x,"
+        "Scale_y,Scale_z,Shear_x,Shear_y,Shear_z,Similarity";
+    if (log_layout_ == LogLayout::Slice)
+        m_logFileId << kCols << ",IteratorCounter\nInitial step\n";
+    else
+        m_logFileId << "Step,Transform[16]," << kCols << ",-log(sim),IteratorCounter\n";
 }
 
-// per-evaluation CSV line (optimizers.cu:709-720)
-void OptimizerBase::log_eval(const std::vector<double>& p, double sim) {
+// One CSV line per evaluation. 3D (optimizers.cu:709-720): step marker, the 16 entries of
+// GetCurrentTransform (the moving-to-fixed matrix), the 12 parameters as floats, LC2,
+// -log(LC2), evaluation count; the starting-pose line of ShowCurrentResults (:613-623) has
+// no -log column. 2D->3D (:1335-1340, :1266-1272): parameters, LC2, evaluation count.
+void OptimizerBase::log_eval(const std::vector<double>& p, double sim, bool initial) {
     if (!m_logFileId.is_open()) return;
-    float m12[12], inv[16];
-    m_logFileId << m_stepMarker << ",";
-    if (lc2e_pose_to_matrix(interp_->pair(), p.data(), (int)p.size(), m12) == 0 &&
-        lc2e::invert_affine(m12, inv)) {
-        if (mdim_ == 4) {
+    if (log_layout_ == LogLayout::Volume) {
+        float m12[12], inv[16];
+        m_logFileId << m_stepMarker << ",";
+        if (lc2e_pose_to_matrix(interp_->pair(), p.data(), (int)p.size(), m12) == 0 &&
+            lc2e::invert_affine(m12, inv))
             for (int i = 0; i < 16; ++i) m_logFileId << inv[i] << ",";
-        } else {
-            const float o[9] = {inv[0], inv[1], inv[3], inv[4], inv[5], inv[7], 0, 0, 1};
-            for (int i = 0; i < 9; ++i) m_logFileId << o[i] << ",";
-        }
     }
     for (double v : p) m_logFileId << (float)v << ",";
-    m_logFileId << sim << "," << -std::log(sim) << "," << EvalCounter << "\n";
+    m_logFileId << sim << ",";
+    if (log_layout_ == LogLayout::Volume && !initial) m_logFileId << -std::log(sim) << ",";
+    m_logFileId << EvalCounter << "\n";
 }
 
 void OptimizerBase::evaluate_batch(int n, int d, const double* X, double* F) {
@@ -647,7 +673,7 @@ void OptimizerBase::ShowCurrentResults() {
     interp_->EvaluateBatch(p, l);
     SimilarityMeasure = l[0];
     log_header();
-    log_eval(p[0], l[0]);
+    log_eval(p[0], l[0], true);
 }
 
 // Optimize (optimizers.cu:183-410, :734-970): DIRECT stage over its entries, then each
@@ -658,9 +684,12 @@ void OptimizerBase::Optimize() {
     // the optimizer's PatchSize (read at optimizers.cu:577) is what it evaluates with (:705)
     interp_->set_patch_size(cfg_.patch_size);
     auto sim_of = [this](double f) { return interp_->patch_size() == 0 ? -f : std::exp(-f); };
+    // optimizers.cu:736 (3D); the 2D optimizer writes no log (the reference's has none), the
+    // 2D->3D log goes next to the other outputs as well (the reference: the working
+    // directory, :1351)
     const std::string logp = interp_->GetOutputPath().empty() ? std::string("./RegistrationInfo.log")
                                                               : interp_->GetOutputPath() + "/RegistrationInfo.log";
-    m_logFileId.open(logp, std::ofstream::out);
+    if (log_layout_ != LogLayout::None) m_logFileId.open(logp, std::ofstream::out);
     m_stepMarker = -1;
     ShowCurrentResults();
     auto obj = [this](int n, int d, const double* X, double* F) { evaluate_batch(n, d, X, F); };
@@ -672,11 +701,27 @@ void OptimizerBase::Optimize() {
         if (!CurrentHolding_OptEntries.empty()) {
             std::vector<double> l, u, x;
             bounds_for(l, u, x);
-            lc2opt::Result r = lc2opt::direct_l(obj, l, u, cfg_.direct_maxeval);
-            if (r.code < 0) std::clog << "optimizer failed: " << r.message << std::endl;
+            if (!DirectLower.empty()) {  // the programmatic configuration's DIRECT bounds
+                int k = 0;
+                for (int e : CurrentHolding_OptEntries)
+                    for (int j = 0; j < entry_width(e); ++j, ++k) {
+                        l[k] = DirectLower[entry_offset(e) + j];
+                        u[k] = DirectUpper[entry_offset(e) + j];
+                    }
+            }
+            // GN_ORIG_DIRECT_L in 3D and 2D->3D (optimizers.cu:854, :1473), GN_DIRECT_L in 2D (:298)
+            lc2opt::opt o(direct_original_ ? lc2opt::GN_ORIG_DIRECT_L : lc2opt::GN_DIRECT_L,
+                          (unsigned)x.size());
+            o.set_batch_objective(obj);
+            o.set_lower_bounds(l);
+            o.set_upper_bounds(u);
+            o.set_maxeval(cfg_.direct_maxeval);
+            double minf = 0.0;
+            const lc2opt::result rc = o.optimize(x, minf);
+            if (rc < 0) std::clog << "optimizer failed: code " << (int)rc << std::endl;
             else {
-                assign_back(r.x);
-                SimilarityMeasure = sim_of(r.f);
+                assign_back(x);
+                SimilarityMeasure = sim_of(minf);
                 std::clog << "Optimization stage-DIRECT finished: similarity -> " << SimilarityMeasure
                           << ", accu evaluations -> " << EvalCounter << std::endl;
             }
@@ -692,13 +737,38 @@ void OptimizerBase::Optimize() {
         if (CurrentHolding_OptEntries.empty()) continue;
         std::vector<double> l, u, x;
         bounds_for(l, u, x);
-        lc2opt::Result r = lc2opt::qmtr(obj, x, l, u, rd.abs_tol, 100000);
-        if (r.code < 0) {
-            std::clog << "optimizer failed: " << r.message << std::endl;
-            continue;
+        double minf = 0.0;
+        if (local_method_ == LocalMethod::Qmtr) {
+            lc2opt::Result r = lc2opt::qmtr(obj, x, l, u, rd.abs_tol, 100000);
+            if (r.code < 0) {
+                std::clog << "optimizer failed: " << r.message << std::endl;
+                continue;
+            }
+            x = r.x;
+            minf = r.f;
+        } else {
+            // nlopt::LN_BOBYQA with set_xtol_rel(AbsTol) and NLopt's default initial step
+            // (optimizers.cu:937-946)
+            lc2opt::opt o(lc2opt::LN_BOBYQA, (unsigned)x.size());
+            o.set_batch_objective(obj);
+            o.set_lower_bounds(l);
+            o.set_upper_bounds(u);
+            o.set_xtol_rel(rd.abs_tol);
+            if (rd.max_eval > 0) o.set_maxeval(rd.max_eval);
+            if (rd.rho_begin > 0.0) {
+                // the programmatic configuration's RhoBegin, as a fraction of each range
+                std::vector<double> dx(x.size());
+                for (size_t i = 0; i < x.size(); ++i) dx[i] = rd.rho_begin * (u[i] - l[i]);
+                o.set_initial_step(dx);
+            }
+            const lc2opt::result rc = o.optimize(x, minf);
+            if (rc < 0) {
+                std::clog << "optimizer failed: code " << (int)rc << std::endl;
+                continue;
+            }
         }
-        assign_back(r.x);
-        SimilarityMeasure = sim_of(r.f);
+        assign_back(x);
+        SimilarityMeasure = sim_of(minf);
         std::clog << "Optimization stage-BOBYQA finished: similarity -> " << SimilarityMeasure
                   << ", accu evaluations -> " << EvalCounter << std::endl;
     }
@@ -757,28 +827,77 @@ void OptimizerBase::ExportResults() {
 
 Optimizers3D::Optimizers3D(std::string FilePath) : OptimizerBase(12, 4) { load_yaml(FilePath); }
 
-// Declared by the reference (optimizers.cuh:102) but never defined there.
-Optimizers3D::Optimizers3D(const LC2Configuration3D* c) : OptimizerBase(12, 4) {
-    if (!c) throw std::runtime_error("null configuration");
-    InitialValues = c->InitialValues;
-    OptimalSolution = InitialValues;
-    LowerBounds = c->LowerBounds;
-    UpperBounds = c->UpperBounds;
-    for (int i = 0; i < 6; ++i) {
-        LowerBounds[i] += InitialValues[i];
-        UpperBounds[i] += InitialValues[i];
+// Optimizers*(const LC2Configuration*): declared by the reference (optimizers.cuh:102, :178)
+// but never defined there. The configuration's optimizer members map onto the YAML's
+// OptimizerSettings as follows (this build's reading of them):
+//   InitialValues = init{Translation, Rotation, Scale, Shear};
+//   bounds (BOBYQA rounds): translation +- initFRE * boundTranslation_BOBYQA (mm), rotation
+//     +- initFRE * boundRotation_BOBYQA (degrees), scale +- 0.2 * boundScale_BOBYQA, shear
+//     +- 0.2 * boundShear_BOBYQA, about the initial values;
+//   DIRECT stage (enabled when MaxEval > 0): translation only, +- initFRE *
+//     boundTranslation_DIRECT, MaxEval evaluations;
+//   BOBYQA round 0: translation and rotation; round 1 (IsAffine only): all parameters; each
+//     with xtol_rel = Tol, at most MaxIterations evaluations and an initial step of RhoBegin
+//     times each range.
+void lc2reg::OptimizerBase::apply_config(const std::vector<double>& initial,
+                                 const std::vector<double>& halfwidth,
+                                 const std::vector<double>& direct_halfwidth, int direct_maxeval,
+                                 bool affine, double rho_begin, double tol, int max_iterations,
+                                 int patch) {
+    InitialValues = initial;
+    OptimalSolution = initial;
+    LowerBounds.resize(nparams_);
+    UpperBounds.resize(nparams_);
+    DirectLower.assign(nparams_, 0.0);
+    DirectUpper.assign(nparams_, 0.0);
+    for (int i = 0; i < nparams_; ++i) {
+        LowerBounds[i] = initial[i] - halfwidth[i];
+        UpperBounds[i] = initial[i] + halfwidth[i];
+        DirectLower[i] = initial[i] - direct_halfwidth[i];
+        DirectUpper[i] = initial[i] + direct_halfwidth[i];
     }
-    cfg_.direct_enabled = c->DirectEnabled;
-    for (int e = 0; e < 4; ++e) cfg_.direct_entries[e] = c->DirectEntries[e];
-    cfg_.direct_maxeval = c->DirectMaxEvaluations;
-    for (auto& r : c->BobyqaRounds) {
+    cfg_.direct_enabled = direct_maxeval > 0;
+    cfg_.direct_entries = {true, false, false, false};
+    cfg_.direct_maxeval = direct_maxeval;
+    cfg_.rounds.clear();
+    for (int k = 0; k < (affine ? 2 : 1); ++k) {
         lc2reg::OptimizerSettings::Round rd;
-        rd.enabled = r.first;
-        rd.entries = r.second;
-        rd.abs_tol = c->BobyqaAbsTol;
+        rd.enabled = true;
+        rd.entries = k == 0 ? std::vector<bool>{true, true, false, false}
+                            : std::vector<bool>{true, true, true, true};
+        rd.abs_tol = tol;
+        rd.max_eval = max_iterations;
+        rd.rho_begin = rho_begin;
         cfg_.rounds.push_back(rd);
     }
-    cfg_.patch_size = c->PatchSize;
+    cfg_.patch_size = patch;
+}
+
+namespace {
+void config_vectors_3d(const LC2Configuration3D* c, std::vector<double>& x, std::vector<double>& h,
+                       std::vector<double>& hd) {
+    const double F = c->initFRE;
+    x = {c->initTranslation.x, c->initTranslation.y, c->initTranslation.z, c->initRotation.x,
+         c->initRotation.y, c->initRotation.z, c->initScale.x, c->initScale.y, c->initScale.z,
+         c->initShear.x, c->initShear.y, c->initShear.z};
+    h = {F * c->boundTranslation_BOBYQA.x, F * c->boundTranslation_BOBYQA.y,
+         F * c->boundTranslation_BOBYQA.z, F * c->boundRotation_BOBYQA.x,
+         F * c->boundRotation_BOBYQA.y, F * c->boundRotation_BOBYQA.z,
+         0.2 * c->boundScale_BOBYQA.x, 0.2 * c->boundScale_BOBYQA.y, 0.2 * c->boundScale_BOBYQA.z,
+         0.2 * c->boundShear_BOBYQA.x, 0.2 * c->boundShear_BOBYQA.y, 0.2 * c->boundShear_BOBYQA.z};
+    hd = h;
+    hd[0] = F * c->boundTranslation_DIRECT.x;
+    hd[1] = F * c->boundTranslation_DIRECT.y;
+    hd[2] = F * c->boundTranslation_DIRECT.z;
+}
+}  // namespace
+
+Optimizers3D::Optimizers3D(const LC2Configuration3D* c) : OptimizerBase(12, 4) {
+    if (!c) throw std::runtime_error("null configuration");
+    std::vector<double> x, h, hd;
+    config_vectors_3d(c, x, h, hd);
+    apply_config(x, h, hd, c->MaxEval, c->IsAffine, c->RhoBegin, c->Tol, (int)c->MaxIterations,
+                 c->PatchSize);
 }
 
 void Optimizers3D::GetOptimalTransform(float* out) {
@@ -795,6 +914,27 @@ Optimizers2D::Optimizers2D(std::string FilePath) : OptimizerBase(6, 3) {
     load_yaml(FilePath);
     direct_original_ = false;        // GN_DIRECT_L (optimizers.cu:298)
     update_initial_on_export_ = false;  // the 2D export keeps the inputs (optimizers.cu:451-480)
+    log_layout_ = LogLayout::None;
+}
+
+// Not declared by the reference; the 2D counterpart of Optimizers3D(const
+// LC2Configuration3D*), with the same mapping of the configuration.
+Optimizers2D::Optimizers2D(const LC2Configuration2D* c) : OptimizerBase(6, 3) {
+    if (!c) throw std::runtime_error("null configuration");
+    const double F = c->initFRE;
+    const std::vector<double> x = {c->initTranslation.x, c->initTranslation.y, c->initRotation,
+                                   c->initScale.x, c->initScale.y, c->initShear};
+    const std::vector<double> h = {F * c->boundTranslation_BOBYQA.x, F * c->boundTranslation_BOBYQA.y,
+                                   F * c->boundRotation_BOBYQA, 0.2 * c->boundScale_BOBYQA.x,
+                                   0.2 * c->boundScale_BOBYQA.y, 0.2 * c->boundShear_BOBYQA};
+    std::vector<double> hd = h;
+    hd[0] = F * c->boundTranslation_DIRECT.x;
+    hd[1] = F * c->boundTranslation_DIRECT.y;
+    apply_config(x, h, hd, c->MaxEval, c->IsAffine, c->RhoBegin, c->Tol, (int)c->MaxIterations,
+                 c->PatchSize);
+    direct_original_ = false;
+    update_initial_on_export_ = false;
+    log_layout_ = LogLayout::None;
 }
 
 void Optimizers2D::GetOptimalTransform(float* out) {
@@ -805,7 +945,19 @@ void Optimizers2D::GetOptimalTransform(float* out) {
 }
 void Optimizers2D::GetOptimalTransform() { GetOptimalTransform(MovingToFixed_Optimal.data()); }
 
-Optimizers2DTo3D::Optimizers2DTo3D(std::string FilePath) : OptimizerBase(12, 4) { load_yaml(FilePath); }
+Optimizers2DTo3D::Optimizers2DTo3D(std::string FilePath) : OptimizerBase(12, 4) {
+    load_yaml(FilePath);
+    log_layout_ = LogLayout::Slice;
+}
+
+Optimizers2DTo3D::Optimizers2DTo3D(const LC2Configuration3D* c) : OptimizerBase(12, 4) {
+    if (!c) throw std::runtime_error("null configuration");
+    std::vector<double> x, h, hd;
+    config_vectors_3d(c, x, h, hd);
+    apply_config(x, h, hd, c->MaxEval, c->IsAffine, c->RhoBegin, c->Tol, (int)c->MaxIterations,
+                 c->PatchSize);
+    log_layout_ = LogLayout::Slice;
+}
 
 void Optimizers2DTo3D::GetOptimalTransform(float* out) {
     static_cast<Interpolator2DTo3D*>(interp_)->GetMovingToFixedTransform(

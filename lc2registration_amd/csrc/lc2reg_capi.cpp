@@ -170,6 +170,38 @@ int lc2reg_evaluate_yaml(const char* mode, const char* yaml_path, const double* 
     }
 }
 
+int lc2reg_minimize_nlopt(int algorithm, int d, const double* x0, const double* lb,
+                          const double* ub, const double* step, double xtol_rel, int maxeval,
+                          lc2reg_batch_fn f, void* user, double* x_out, double* f_out,
+                          int* nevals) {
+    if (d <= 0 || !lb || !ub || !f || algorithm < 0 || algorithm > 2 || (algorithm == 2 && !x0)) {
+        g_err = "lc2reg_minimize_nlopt: invalid argument";
+        return -2;
+    }
+    try {
+        lc2opt::opt o((lc2opt::algorithm)algorithm, (unsigned)d);
+        o.set_batch_objective(wrap(f, user));
+        o.set_lower_bounds(std::vector<double>(lb, lb + d));
+        o.set_upper_bounds(std::vector<double>(ub, ub + d));
+        o.set_xtol_rel(xtol_rel);
+        if (maxeval > 0) o.set_maxeval(maxeval);
+        if (step) o.set_initial_step(std::vector<double>(step, step + d));
+        std::vector<double> x = x0 ? std::vector<double>(x0, x0 + d) : std::vector<double>(d, 0.0);
+        double fx = 0.0;
+        const lc2opt::result rc = o.optimize(x, fx);
+        if (nevals) *nevals = o.get_numevals();
+        if (rc < 0) {
+            g_err = "lc2reg_minimize_nlopt: optimizer failed";
+            return (int)rc;
+        }
+        if (x_out) std::memcpy(x_out, x.data(), d * sizeof(double));
+        if (f_out) *f_out = fx;
+        return (int)rc;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
 int lc2reg_minimize_direct(int d, const double* lb, const double* ub, int maxeval,
                            lc2reg_batch_fn f, void* user, double* x_out, double* f_out,
                            int* nevals) {

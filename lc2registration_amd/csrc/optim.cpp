@@ -21,7 +21,7 @@ int min_level(const Box& b) { return *std::min_element(b.lev.begin(), b.lev.end(
 }  // namespace
 
 Result direct_l(const BatchObjective& f, const std::vector<double>& lb,
-                const std::vector<double>& ub, int maxeval, double eps) {
+                const std::vector<double>& ub, int maxeval, double eps, bool original) {
     Result res;
     const int n = (int)lb.size();
     if (n == 0 || ub.size() != lb.size()) {
@@ -96,7 +96,7 @@ Result direct_l(const BatchObjective& f, const std::vector<double>& lb,
             Split sp{s, {}};
             for (int i = 0; i < n; ++i)
                 if (boxes[s].lev[i] == L) sp.dims.push_back(i);
-            if ((int)sp.dims.size() * 2 > budget) break;
+            if (!original && (int)sp.dims.size() * 2 > budget) break;
             budget -= (int)sp.dims.size() * 2;
             const double d = std::pow(3.0, -(L + 1));
             for (int i : sp.dims)
@@ -301,6 +301,45 @@ Result qmtr(const BatchObjective& f, const std::vector<double>& x0, const std::v
     for (int i = 0; i < n; ++i) res.x[i] = y[i] * s[i];
     res.f = f0;
     return res;
+}
+
+void opt::set_objective(vfunc f, void* data, bool maximize) {
+    maximize_ = maximize;
+    batch_ = [f, data](int k, int d, const double* X, double* F) {
+        std::vector<double> grad;  // derivative-free: NLopt passes an empty gradient
+        for (int j = 0; j < k; ++j) {
+            std::vector<double> x(X + (size_t)j * d, X + (size_t)j * d + d);
+            F[j] = f(x, grad, data);
+        }
+    };
+}
+
+result opt::optimize(std::vector<double>& x, double& opt_f) {
+    nevals_ = 0;
+    if (!batch_ || lb_.size() != n_ || ub_.size() != n_ || (alg_ == LN_BOBYQA && x.size() != n_))
+        return INVALID_ARGS;
+    const bool mx = maximize_;
+    const BatchObjective inner = batch_;
+    // minimise -f for a max objective, as NLopt does
+    BatchObjective obj = [inner, mx](int k, int d, const double* X, double* F) {
+        inner(k, d, X, F);
+        if (mx)
+            for (int j = 0; j < k; ++j) F[j] = -F[j];
+    };
+    Result r;
+    if (alg_ == LN_BOBYQA) {
+        const std::vector<double> step =
+            step_.size() == n_ ? step_ : default_initial_step(x, lb_, ub_);
+        r = bobyqa(obj, x, lb_, ub_, step, xtol_rel_, maxeval_);
+    } else {
+        if (maxeval_ <= 0) return INVALID_ARGS;  // DIRECT needs a budget here
+        r = direct_l(obj, lb_, ub_, maxeval_, 1e-4, alg_ == GN_ORIG_DIRECT_L);
+    }
+    nevals_ = r.nevals;
+    if (r.code < 0) return r.code == -2 ? INVALID_ARGS : FAILURE;
+    x = r.x;
+    opt_f = mx ? -r.f : r.f;
+    return r.code == 5 ? MAXEVAL_REACHED : r.code == 4 ? XTOL_REACHED : SUCCESS;
 }
 
 }  // namespace lc2opt

@@ -36,6 +36,8 @@ def lib():
                                              DP, C.POINTER(C.c_int)]
         L.lc2reg_minimize_qmtr.argtypes = [C.c_int, DP, DP, DP, C.c_double, C.c_int, BATCH_FN,
                                            C.c_void_p, DP, DP, C.POINTER(C.c_int)]
+        L.lc2reg_minimize_nlopt.argtypes = [C.c_int, C.c_int, DP, DP, DP, DP, C.c_double, C.c_int,
+                                            BATCH_FN, C.c_void_p, DP, DP, C.POINTER(C.c_int)]
         L.lc2reg_yaml_roundtrip.argtypes = [C.c_char_p, C.c_char_p]
         _lib = L
     return _lib
@@ -102,6 +104,27 @@ def minimize_qmtr(fn, x0, lb, ub, xtol: float = 1e-4, maxeval: int = 100000):
     _check(lib().lc2reg_minimize_qmtr(x0.size, _p(x0), _p(lb), _p(ub), xtol, maxeval, cb, None,
                                       _p(x), C.byref(f), C.byref(n)))
     return x, f.value, n.value
+
+
+GN_DIRECT_L, GN_ORIG_DIRECT_L, LN_BOBYQA = 0, 1, 2
+
+
+def minimize_nlopt(algorithm: int, fn, lb, ub, x0=None, xtol_rel: float = 0.0, maxeval: int = 0,
+                   step=None):
+    """The nlopt::opt-shaped facade the Optimizers* classes use (GN_DIRECT_L,
+    GN_ORIG_DIRECT_L, LN_BOBYQA); fn as for minimize_direct. Returns (x, f, evaluations,
+    NLopt result code)."""
+    lb, ub = (np.ascontiguousarray(v, np.float64) for v in (lb, ub))
+    x0 = np.ascontiguousarray(lb if x0 is None else x0, np.float64)
+    st = None if step is None else np.ascontiguousarray(step, np.float64)
+    x, f, n = np.zeros(lb.size), C.c_double(), C.c_int()
+    cb = _objective(fn)
+    rc = lib().lc2reg_minimize_nlopt(algorithm, lb.size, _p(x0), _p(lb), _p(ub),
+                                     None if st is None else _p(st), xtol_rel, maxeval, cb, None,
+                                     _p(x), C.byref(f), C.byref(n))
+    if rc < 0:
+        raise RegistrationError(lib().lc2reg_last_error().decode())
+    return x, f.value, n.value, rc
 
 
 def _num(v):

@@ -111,7 +111,7 @@ def test_yaml_registration_3d_recovers_known_pose(reg, tmp_path):
     assert np.allclose([opt["InitialValues"][k] for k in KEYS_3D], x.astype(np.float32), atol=1e-6)
     assert opt["BOBYQA"][0]["Enabled"] is False and opt["BOBYQA"][1]["Enabled"] is True
     log = open(tmp_path / "RegistrationInfo.log").read().splitlines()
-    assert log[0].startswith("Step,Transform[16],TX,") and len(log) == nev + 2
+    assert log[0].startswith("Step,Transform[16],Translation_x,") and len(log) == nev + 2
     assert os.path.exists(tmp_path / "MovingToFixed.mat")
     assert os.path.exists(tmp_path / "fixed.nrrd") and os.path.exists(tmp_path / "moving_Registered.nrrd")
 
