@@ -100,7 +100,12 @@ void lc2e_volume_destroy(lc2e_volume* v);
  *  high_res_is_moving: HighResolutionModality "Moving" (1) or "Fixed" (0), interpolator.cu:590-605.
  *  dim: LC2 grid dimensionality; 2 with a 3D moving volume is the 2D->3D mode
  *       (Interpolator2DTo3D, interpolator.cu:1538-1786); a fixed image with dims[2] > 1 and
- *       dim 2 is a stack of independent 2D frames. */
+ *       dim 2 is a stack of independent 2D frames.
+ *  patch_radius: 1..64. Radii up to 4 (3D) / 10 (2D) run the fused kernels; larger radii run
+ *       a separable path (the reference's global-memory kernel for large patches,
+ *       cu_lc2_3d.cu:769-827), whose buffers -- the moving image and its |Sobel| on the grid
+ *       and two f64 box-sum buffers of 7 (moving high-resolution image) or 10 channels -- are
+ *       allocated here, against the budget. */
 int lc2e_pair_create(lc2e_engine* e, lc2e_volume* fixed, lc2e_volume* moving,
                      const float resampled_spacing[3], int high_res_is_moving,
                      int center_overlaid, int patch_radius, int dim, lc2e_pair** out);
@@ -174,13 +179,16 @@ int lc2e_pair_autotune(lc2e_pair* p, int n, const float* mats, int iters, int* b
                        double* best_ms);
 
 /* Kernel timing with HIP events recorded around every fused-kernel launch on the engine's
- * stream (eager path only). enable = 1 resets the counters. read returns the summed
- * fused-kernel time in ms and the number of launches since the reset. */
+ * stream (in a HIP graph as event-record nodes, timed on every replay). enable = 1 resets
+ * the counters. read returns the summed fused-kernel time in ms and the number of launches
+ * since the reset. */
 int lc2e_pair_profile(lc2e_pair* p, int enable);
 int lc2e_pair_profile_read(lc2e_pair* p, double* kernel_ms, long long* launches);
 
-/* Record the batched evaluation of n poses in a HIP graph and replay it for later
- * lc2e_eval_batch calls with the same n (1 = on, 0 = off). */
+/* Record the batched evaluation of n poses (matrix upload, kernels, result download) in a
+ * HIP graph on its first lc2e_eval_batch with n poses and replay it for later calls with the
+ * same n (1 = on, 0 = off). Graphs are dropped when the pair's buffers or launch shape
+ * change. The registration facade turns this on for its pairs. */
 int lc2e_pair_use_graph(lc2e_pair* p, int enable);
 
 #ifdef __cplusplus

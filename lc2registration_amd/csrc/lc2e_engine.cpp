@@ -2,6 +2,7 @@
 // evaluation on one HIP stream, HBM budget accounting, optional HIP-graph replay.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -123,6 +124,10 @@ struct lc2e_pair {
     size_t cap_bytes = 0;
     bool use_graph = false;
     std::map<int, GraphEntry> graphs;
+    // changes whenever captured launches of this pair go stale (buffers or decomposition
+    // changed); unique across pairs, so a graph cached by address is never reused for
+    // another pair
+    unsigned long long gen = 0;
     // mutual information (patchSize 0): extremes of the static image, per-pose extremes,
     // joint histograms and results, sized for mi_cap poses of mi_bins counters
     int* mi_ext_stat = nullptr;
@@ -133,6 +138,12 @@ struct lc2e_pair {
     unsigned* mi_hist = nullptr;
     double* mi_out = nullptr;
     int resident_per_cu = 0;  // fused work-groups one CU holds at once (0: not queried yet)
+    // separable path (radius beyond the fused kernels): grids and box-sum buffers, made at
+    // pair creation
+    bool sep = false;
+    float* sep_grids = nullptr;    // [2][V] f32: moving image, its |Sobel|
+    double* sep_sums = nullptr;    // [2][channels][V] f64
+    size_t sep_grid_bytes = 0, sep_sum_bytes = 0;
     // kernel timing (eager path)
     bool profile = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -144,12 +155,15 @@ struct lc2e_pair {
 
 namespace {
 
+std::atomic<unsigned long long> g_generation{1};
+
 void drop_graphs(lc2e_pair* p) {
     for (auto& kv : p->graphs) {
         if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
         if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
     }
     p->graphs.clear();
+    p->gen = g_generation.fetch_add(1);
 }
 
 // Work decomposition of one launch.
@@ -163,8 +177,19 @@ int round_zchunk(int c) { return (c + lc2e::kZBlock - 1) / lc2e::kZBlock * lc2e:
 
 lc2e::FusedArgs make_args(lc2e_pair* p, const float* d_mats, double* d_partials, const Tiling& t);
 
+// The pair's evaluation kernels for n poses: the fused kernel, or the separable path for
+// radii beyond it.
+hipError_t launch_eval(lc2e_pair* p, const lc2e::FusedArgs& a, int n, hipStream_t st) {
+    if (p->sep)
+        return lc2e::launch_sep(a, p->info.grid_ndim, p->info.src_ndim, p->info.high_res_is_moving, n,
+                                st);
+    return lc2e::launch_fused(a, p->info.grid_ndim, p->info.src_ndim, p->info.high_res_is_moving, n,
+                              st);
+}
+
 // Fused work-groups one CU holds at once for this pair's kernel (queried once, then kept).
 int resident_per_cu(lc2e_pair* p) {
+    if (p->sep) return 1;  // the separable path's launches fill the device by themselves
     if (p->resident_per_cu <= 0) {
         const lc2e::FusedArgs a = make_args(p, nullptr, nullptr, Tiling{});
         int per_cu = 0;
@@ -207,6 +232,10 @@ int model_zchunk(lc2e_pair* p, int n) {
 Tiling tiling_for(lc2e_pair* p, int n) {
     Tiling t;
     const int r = p->info.patch_radius;
+    if (p->sep) {  // one partial per 256 output voxels
+        t.tiles = t.bpp = lc2e::sep_blocks(p->v, r, p->info.grid_ndim);
+        return t;
+    }
     if (p->info.grid_ndim == 3) {
         auto it = p->tuned.find(n);
         t.zchunk = round_zchunk(p->zchunk_forced > 0 ? p->zchunk_forced
@@ -319,6 +348,13 @@ lc2e::FusedArgs make_args(lc2e_pair* p, const float* d_mats, double* d_partials,
     a.blocks_per_pose = t.bpp;
     a.tiles = t.tiles;
     a.mats_per_pose = p->mats_per_pose();
+    if (p->sep) {
+        const size_t V = p->vbytes / sizeof(float);
+        a.sep_img = p->sep_grids;
+        a.sep_grad = p->sep_grids + V;
+        a.sep_a = p->sep_sums;
+        a.sep_b = p->sep_sums + (size_t)lc2e::kSepChannels(p->info.high_res_is_moving != 0) * V;
+    }
     return a;
 }
 
@@ -329,10 +365,10 @@ int enqueue_batch(lc2e_pair* p, int n, const Tiling& t, hipStream_t st) {
                            hipMemcpyHostToDevice, st));
     if (t.tiles > 0) {
         lc2e::FusedArgs a = make_args(p, p->d_mats, p->d_partials, t);
-        const bool timed = p->profile && !p->use_graph;
+        // in a captured graph the events become event-record nodes, timed on every replay
+        const bool timed = p->profile;
         if (timed) HIP_TRY(hipEventRecord(p->ev0, st));
-        HIP_TRY(lc2e::launch_fused(a, p->info.grid_ndim, p->info.src_ndim,
-                                   p->info.high_res_is_moving, n, st));
+        HIP_TRY(launch_eval(p, a, n, st));
         if (timed) HIP_TRY(hipEventRecord(p->ev1, st));
         HIP_TRY(lc2e::launch_reduce(p->d_partials, t.bpp, n, p->d_out, st));
     } else {
@@ -370,7 +406,7 @@ int run_batch(lc2e_pair* p, int n, const Tiling& t) {
         if (rc < 0) return rc;
     }
     HIP_TRY(hipStreamSynchronize(st));
-    if (p->profile && !p->use_graph && t.tiles > 0) {
+    if (p->profile && t.tiles > 0) {
         float ms = 0.0f;
         HIP_TRY(hipEventElapsedTime(&ms, p->ev0, p->ev1));
         p->prof_ms += ms;
@@ -392,6 +428,14 @@ int engine_device(lc2e_engine* e) { return e->device; }
 // The share of the device one fused launch of n poses occupies: its work-groups over the
 // work-groups all CUs hold at once (>= 1: it fills the GPU by itself). The serving loop
 // runs launches side by side only while their shares add up to at most 1.
+// Generation of the pair's launch state: graphs captured under another generation are stale.
+unsigned long long pair_generation(lc2e_pair* p) { return p->gen; }
+// Make the pair's batch buffers fit n poses now, so that a following
+// lc2e_eval_batch_async of n poses allocates nothing (e.g. under stream capture).
+int pair_prepare(lc2e_pair* p, int n) {
+    const Tiling t = tiling_for(p, n);
+    return ensure_capacity(p, n, t.bpp);
+}
 double launch_share(lc2e_pair* p, int n) {
     const Tiling t = tiling_for(p, n);
     if (t.tiles <= 0 || n <= 0) return 0.0;
@@ -533,10 +577,10 @@ int lc2e_pair_create(lc2e_engine* e, lc2e_volume* fixed, lc2e_volume* moving,
     if (fixed->e != e || moving->e != e) return fail(LC2E_ERR_INVALID, "volume of another engine");
     if (dim != 2 && dim != 3) return fail(LC2E_ERR_INVALID, "dim must be 2 or 3");
     if (patch_radius < 1) return fail(LC2E_ERR_INVALID, "patch radius must be >= 1 (0 selects MI, unsupported)");
-    if (dim == 3 && patch_radius > lc2e::kMaxR3)
-        return fail(LC2E_ERR_UNSUPPORTED, "3D patch radius %d > %d", patch_radius, lc2e::kMaxR3);
-    if (dim == 2 && patch_radius > lc2e::kMaxR2)
-        return fail(LC2E_ERR_UNSUPPORTED, "2D patch radius %d > %d", patch_radius, lc2e::kMaxR2);
+    if (patch_radius > lc2e::kMaxRSep)
+        return fail(LC2E_ERR_UNSUPPORTED, "patch radius %d > %d", patch_radius, lc2e::kMaxRSep);
+    // radii beyond the fused kernels take the separable path
+    const bool sep = (dim == 3 && patch_radius > lc2e::kMaxR3) || (dim == 2 && patch_radius > lc2e::kMaxR2);
     const int src_ndim = moving->g.dims[2] > 1 ? 3 : 2;
     if (dim == 3 && (src_ndim != 3 || fixed->g.dims[2] < 1))
         return fail(LC2E_ERR_INVALID, "3D LC2 needs a 3D moving volume");
@@ -549,6 +593,7 @@ int lc2e_pair_create(lc2e_engine* e, lc2e_volume* fixed, lc2e_volume* moving,
     if (V >= (1LL << 31)) return fail(LC2E_ERR_UNSUPPORTED, "virtual grid exceeds 2^31 voxels");
 
     auto* p = new lc2e_pair();
+    p->gen = g_generation.fetch_add(1);
     p->e = e;
     p->fixed = fixed;
     p->moving = moving;
@@ -574,6 +619,18 @@ int lc2e_pair_create(lc2e_engine* e, lc2e_volume* fixed, lc2e_volume* moving,
         delete p;
         return rc;
     }
+    if (sep) {
+        p->sep = true;
+        p->sep_grid_bytes = 2 * (size_t)V * sizeof(float);
+        p->sep_sum_bytes = 2 * (size_t)lc2e::kSepChannels(high_res_is_moving != 0) * V * sizeof(double);
+        if ((rc = e->alloc((void**)&p->sep_grids, p->sep_grid_bytes)) < 0 ||
+            (rc = e->alloc((void**)&p->sep_sums, p->sep_sum_bytes)) < 0) {
+            if (p->sep_grids) e->release(p->sep_grids, p->sep_grid_bytes);
+            e->release(p->stat_img, p->vbytes);
+            delete p;
+            return rc;
+        }
+    }
     const lc2e_geom& fg = fixed->g;
     lc2e::SGeom fsg = lc2e::make_sgeom(fg.dims, fg.spacing, fg.origin);
     hipError_t err = lc2e::launch_warp_static(fixed->d, fixed->f16, fsg, p->v, dim, src_ndim,
@@ -582,8 +639,7 @@ int lc2e_pair_create(lc2e_engine* e, lc2e_volume* fixed, lc2e_volume* moving,
         // the fixed image is the ultrasound: its patch statistics are pose-independent
         rc = e->alloc((void**)&p->ustat, 2 * (size_t)V * sizeof(double));
         if (rc < 0) {
-            e->release(p->stat_img, p->vbytes);
-            delete p;
+            lc2e_pair_destroy(p);
             return rc;
         }
         err = lc2e::launch_ustats(p->stat_img, p->v, dim, patch_radius, p->ustat, p->ustat + V,
@@ -593,8 +649,7 @@ int lc2e_pair_create(lc2e_engine* e, lc2e_volume* fixed, lc2e_volume* moving,
         // the fixed image is the high-resolution one: its |Sobel| is pose-independent
         rc = e->alloc((void**)&p->stat_grad, p->vbytes);
         if (rc < 0) {
-            e->release(p->stat_img, p->vbytes);
-            delete p;
+            lc2e_pair_destroy(p);
             return rc;
         }
         err = lc2e::launch_sobel(p->stat_img, p->v, dim, p->stat_grad, e->stream);
@@ -619,6 +674,8 @@ void lc2e_pair_destroy(lc2e_pair* p) {
     p->e->release(p->stat_img, p->vbytes);
     if (p->stat_grad) p->e->release(p->stat_grad, p->vbytes);
     if (p->ustat) p->e->release(p->ustat, 2 * (p->vbytes / sizeof(float)) * sizeof(double));
+    if (p->sep_grids) p->e->release(p->sep_grids, p->sep_grid_bytes);
+    if (p->sep_sums) p->e->release(p->sep_sums, p->sep_sum_bytes);
     delete p;
 }
 
@@ -684,8 +741,7 @@ int lc2e_eval_batch_async(lc2e_pair* p, int n, const float* d_mats, double* d_ou
     if (rc < 0) return rc;
     if (t.tiles > 0) {
         lc2e::FusedArgs a = make_args(p, d_mats, p->d_partials, t);
-        HIP_TRY(lc2e::launch_fused(a, p->info.grid_ndim, p->info.src_ndim,
-                                   p->info.high_res_is_moving, n, st));
+        HIP_TRY(launch_eval(p, a, n, st));
     }
     HIP_TRY(lc2e::launch_reduce(p->d_partials, t.tiles > 0 ? t.bpp : 0, n, d_out, st));
     return LC2E_OK;
@@ -795,7 +851,7 @@ int lc2e_debug_fetch(lc2e_pair* p, lc2e_stage stage, const float* m12, void* hos
     a.wmap = maps;
     a.mmap = maps + V;
     if (err == hipSuccess && t.tiles > 0)
-        err = lc2e::launch_fused(a, p->info.grid_ndim, p->info.src_ndim, p->info.high_res_is_moving, 1, st);
+        err = launch_eval(p, a, 1, st);
     if (err == hipSuccess)
         err = hipMemcpyAsync(host, stage == LC2E_STAGE_W ? a.wmap : a.mmap, V * sizeof(double),
                              hipMemcpyDeviceToHost, st);
@@ -826,7 +882,7 @@ int lc2e_pair_autotune(lc2e_pair* p, int n, const float* mats, int iters, int* b
                        double* best_ms) {
     if (!p || !mats || n <= 0 || iters <= 0) return fail(LC2E_ERR_INVALID, "invalid argument");
     HIP_TRY(hipSetDevice(p->e->device));
-    if (p->info.grid_ndim != 3) {
+    if (p->info.grid_ndim != 3 || p->sep) {
         if (best_zchunk) *best_zchunk = 1;
         if (best_ms) *best_ms = 0.0;
         return LC2E_OK;
@@ -891,6 +947,10 @@ int lc2e_pair_get_tuning(lc2e_pair* p, int* zchunk) {
 
 int lc2e_pair_profile(lc2e_pair* p, int enable) {
     if (!p) return fail(LC2E_ERR_INVALID, "null pair");
+    if ((enable != 0) != p->profile) {
+        (void)hipStreamSynchronize(p->e->stream);
+        drop_graphs(p);  // graphs are captured with or without the timing events
+    }
     if (enable && !p->ev0) {
         HIP_TRY(hipEventCreate(&p->ev0));
         HIP_TRY(hipEventCreate(&p->ev1));

@@ -707,6 +707,11 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
     const int nsteps = nout + 2 * R + LAG;
 
     Pending3<T> pend[SPT];
+    // A wave whose 64 positions of sample j all lie past the warped region skips that sample
+    // (its gathers and lerps): with 576 positions on 256 threads the third sample belongs to
+    // wave 0 alone. The test is wave-uniform (scalar branch).
+    const int wbase = __builtin_amdgcn_readfirstlane(tid & ~63);
+    auto wave_has = [&](int j, int n) { return (j + 1) * NTHR <= n || j * NTHR + wbase < n; };
     // the in-plane part of each warped position's source coordinates, m*(Lx, Ly, 0), is the
     // same for every plane: computed once, in the order issue3 uses
     float cxy[SPT][3];
@@ -727,7 +732,7 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
         const float lz = fmaf((float)zi, a.v.sz, a.v.oz);
 #pragma unroll
         for (int j = 0; j < SPT; ++j)
-            issue3c(src, a.s, m, cxy[j], lz, zin && ((in_xy >> j) & 1u), pend[j]);
+            if (wave_has(j, NI)) issue3c(src, a.s, m, cxy[j], lz, zin && ((in_xy >> j) & 1u), pend[j]);
     };
     // static image values (U if MOVI, else I and |grad I|) of the products positions
     float pre0[PPT], pre1[PPT];
@@ -796,6 +801,7 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
             __builtin_amdgcn_s_setprio(2);
 #pragma unroll
             for (int j = 0; j < SPT; ++j) {
+                if (!wave_has(j, NI)) continue;
                 const int i = tid + j * NTHR;
                 const float v = finish3(pend[j]);
                 if (i < NI) s_in[i] = v;
@@ -814,6 +820,7 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
             // in registers along z, |grad| and the products of plane zp
 #pragma unroll
             for (int j = 0; j < PPT; ++j) {
+                if (!wave_has(j, NP)) continue;
                 const int i = tid + j * NTHR;
                 if (i < NP) {
                     const int lx = i % PX, ly = i / PX;
@@ -997,6 +1004,9 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
 
     using Pend = typename std::conditional<SRC3D, Pending3<T>, Pending2<T>>::type;
     Pend pend[SPT];
+    // waves whose positions of sample / products position j all lie past the region skip
+    // it (wave-uniform, as in the 3D kernel)
+    const int wbase = __builtin_amdgcn_readfirstlane(tid & ~63);
     auto issue_frame = [&](int frame) {
         float m[12];
         const float* mp = mats + (a.mats_per_pose > 1 ? frame : 0) * 12;
@@ -1005,6 +1015,7 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
 #pragma unroll
         for (int j = 0; j < SPT; ++j) {
             if (j * kThreads >= NI) break;  // uniform
+            if (j * kThreads + wbase >= NI) continue;  // wave-uniform
             const bool inside = (g_in >> j) & 1u;
             if constexpr (SRC3D) issue3(src, a.s, m, gLx[j], gLy[j], lz, inside, pend[j]);
             else issue2(src, a.s, m, gLx[j], gLy[j], inside, pend[j]);
@@ -1045,6 +1056,7 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
 #pragma unroll
         for (int j = 0; j < SPT; ++j) {
             if (j * kThreads >= NI) break;  // uniform
+            if (j * kThreads + wbase >= NI) continue;  // wave-uniform
             const int i = tid + j * kThreads;
             float v;
             if constexpr (SRC3D) v = finish3(pend[j]);
@@ -1068,6 +1080,7 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
             if (j * kThreads >= NP) break;  // uniform
+            if (j * kThreads + wbase >= NP) continue;  // wave-uniform
             const int i = tid + j * kThreads;
             if (i < NP) {
                 float I, G, U;
@@ -1123,6 +1136,158 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
     }
     double* dst = a.partials + ((size_t)pose * a.blocks_per_pose + (size_t)run * cols + tcol) * 2;
     block_reduce_store<kThreads>(acc_m, acc_w, s_red, dst);
+}
+
+// ---------------------------------------------------------------------------
+// Separable path for patch radii beyond the fused kernels (3D r > kMaxR3, 2D r > kMaxR2):
+// the reference switches to a global-memory kernel there (cu_lc2_3d.cu:769-827, chosen at
+// :1133-1177). Here, per pose: the moving image is warped (and, if it is the high-resolution
+// one, its |Sobel| taken) into two f32 grids, then the patch sums of every moment channel are
+// box sums along x, y (and z) through two f64 buffers, each a van Herk / Gil-Werman pass
+// (exact in the sense of the fused kernels: no partial sum holds a value from outside its
+// window), and a last kernel combines them per voxel and reduces per work-group.
+// Channels: moving high-resolution image (7): I, G, II, GG, IG, UI, UG; moving ultrasound
+// (10): I, G, U, II, GG, IG, UU, UI, UG and the count of U > 1e-6 (exact as a double).
+// ---------------------------------------------------------------------------
+template <bool MOVI>
+__device__ __forceinline__ double sep_product(int c, float I, float G, float U) {
+    const double di = I, dg = G, du = U;
+    if (MOVI) {
+        switch (c) {
+            case 0: return di;
+            case 1: return dg;
+            case 2: return di * di;
+            case 3: return dg * dg;
+            case 4: return di * dg;
+            case 5: return du * di;
+            default: return du * dg;
+        }
+    }
+    switch (c) {
+        case 0: return di;
+        case 1: return dg;
+        case 2: return du;
+        case 3: return di * di;
+        case 4: return dg * dg;
+        case 5: return di * dg;
+        case 6: return du * du;
+        case 7: return du * di;
+        case 8: return du * dg;
+        default: return (du > kZeroThreshold) ? 1.0 : 0.0;
+    }
+}
+
+// Window sums of width W along one line of n inputs: out(s) = in(s) + ... + in(s + W - 1)
+// for s < n - W + 1, van Herk form with the output itself holding the block suffix sums
+// between the two passes (no per-thread storage of W values).
+template <class In, class Out>
+__device__ __forceinline__ void vh_line(int n, int W, In in, Out out) {
+    const int nw = n - W + 1;
+    for (int b = 0; b < nw; b += W) {  // suffix sums of each block, from its end
+        double acc = 0.0;
+        for (int i = min(b + W, n) - 1; i >= b; --i) {
+            acc += in(i);
+            if (i < nw) out(i) = acc;
+        }
+    }
+    for (int b = 0; b < nw; b += W) {  // plus the prefix of the next block
+        double pre = 0.0;
+        for (int j = 0; j < W && b + j < nw; ++j) {
+            if (j > 0) out(b + j) = out(b + j) + pre;
+            if (b + W + j < n) pre += in(b + W + j);
+        }
+    }
+}
+
+// One box pass. axis 0 reads the products of the I, G, U grids (U: ultrasound on the grid,
+// I: high-resolution image, G: its |Sobel|); axes 1 and 2 read `src`. Window sums of lines
+// along `axis` go to `dst` at the index of the window start. Buffers are [channel][V].
+template <bool MOVI>
+__global__ __launch_bounds__(256) void k_sep_pass(const float* __restrict__ I,
+                                                  const float* __restrict__ G,
+                                                  const float* __restrict__ U,
+                                                  const double* __restrict__ src,
+                                                  double* __restrict__ dst, VGeom v, int W,
+                                                  int axis) {
+    constexpr int NC = MOVI ? 7 : 10;
+    const long long V = (long long)v.X * v.Y * v.Z;
+    const int nX = axis == 0 ? 1 : v.X - W + 1;  // x positions of the lines (window starts)
+    const int nY = axis == 2 ? v.Y - W + 1 : (axis == 1 ? 1 : v.Y);
+    const long long lines = axis == 0 ? (long long)v.Y * v.Z
+                          : axis == 1 ? (long long)nX * v.Z
+                                      : (long long)nX * nY;
+    const long long tasks = lines * NC;
+    for (long long t = blockIdx.x * 256LL + threadIdx.x; t < tasks; t += (long long)gridDim.x * 256) {
+        const int c = (int)(t / lines);
+        const long long l = t % lines;
+        long long base;  // index of the line's first element
+        long long stride;
+        int n;
+        if (axis == 0) {
+            base = l * v.X;
+            stride = 1;
+            n = v.X;
+        } else if (axis == 1) {
+            const int x = (int)(l % nX), z = (int)(l / nX);
+            base = (long long)z * v.X * v.Y + x;
+            stride = v.X;
+            n = v.Y;
+        } else {
+            const int x = (int)(l % nX), y = (int)(l / nX);
+            base = (long long)y * v.X + x;
+            stride = (long long)v.X * v.Y;
+            n = v.Z;
+        }
+        double* o = dst + (long long)c * V + base;
+        if (axis == 0) {
+            vh_line(n, W, [&](int i) { return sep_product<MOVI>(c, I[base + i], G[base + i], U[base + i]); },
+                    [&](int s) -> double& { return o[s]; });
+        } else {
+            const double* in = src + (long long)c * V + base;
+            vh_line(n, W, [&](int i) { return in[i * stride]; },
+                    [&](int s) -> double& { return o[s * stride]; });
+        }
+    }
+}
+
+// Per output voxel: the patch sums (at the window-start index of `S`), weight and measure;
+// per work-group sums into the pose's partials (fixed positions, fixed order).
+template <bool MOVI>
+__global__ __launch_bounds__(256) void k_sep_combine(const double* __restrict__ S,
+                                                     const double* __restrict__ ustat_mean,
+                                                     const double* __restrict__ ustat_cyy,
+                                                     VGeom v, int r, int grid3, double* partials,
+                                                     double* wmap, double* mmap) {
+    __shared__ double s_red[8];
+    constexpr int NC = MOVI ? 7 : 10;
+    const long long V = (long long)v.X * v.Y * v.Z;
+    const int W = 2 * r + 1;
+    const int ex = v.X - 2 * r, ey = v.Y - 2 * r, ez = grid3 ? v.Z - 2 * r : v.Z;
+    const int N = grid3 ? W * W * W : W * W;
+    const long long nout = (long long)ex * ey * ez;
+    const long long o = blockIdx.x * 256LL + threadIdx.x;
+    double am = 0.0, aw = 0.0;
+    if (o < nout) {
+        const int sx = (int)(o % ex), sy = (int)((o / ex) % ey), sz = (int)(o / ((long long)ex * ey));
+        const long long si = ((long long)sz * v.Y + sy) * v.X + sx;           // window start
+        const long long vo = ((long long)(sz + (grid3 ? r : 0)) * v.Y + (sy + r)) * v.X + (sx + r);
+        double Sc[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) Sc[c] = S[(long long)c * V + si];
+        double w, meas;
+        bool ok;
+        if (MOVI) ok = combine_static_u(Sc, ustat_mean[vo], ustat_cyy[vo], N, w, meas);
+        else ok = combine(Sc, (int)Sc[9], N, w, meas);
+        if (ok) {
+            am = meas;
+            aw = w;
+            if (wmap) {
+                wmap[vo] = w;
+                mmap[vo] = meas;
+            }
+        }
+    }
+    block_reduce_store<256>(am, aw, s_red, partials + (size_t)blockIdx.x * 2);
 }
 
 // Per-pose fixed-order reduction of the work-group partials -> (sum m, sum w, LC2)
@@ -1603,18 +1768,71 @@ hipError_t fused_resident_per_cu(const FusedArgs& a, int grid_ndim, int src_ndim
                      : launch2t<float>(a, movi, src_ndim, dim3(1), nullptr, per_cu);
 }
 
-hipError_t launch_reduce(const double* partials, int blocks_per_pose, int n_poses, double* out,
-                         hipStream_t st) {
-    hipLaunchKernelGGL(k_reduce, dim3(n_poses), dim3(256), 0, st, partials, blocks_per_pose, out);
-    return hipGetLastError();
-}
-
 static dim3 grid_for(long long n) {
     long long b = (n + 255) / 256;
     if (b > 65535LL * 16) b = 65535LL * 16;
     if (b < 1) b = 1;
     return dim3((unsigned)b);
 }
+
+int sep_blocks(const VGeom& v, int r, int grid_ndim) {
+    const long long ex = v.X - 2 * r, ey = v.Y - 2 * r, ez = grid_ndim == 3 ? v.Z - 2 * r : v.Z;
+    if (ex <= 0 || ey <= 0 || ez <= 0) return 0;
+    return (int)((ex * ey * ez + 255) / 256);
+}
+
+hipError_t launch_sep(const FusedArgs& a, int grid_ndim, int src_ndim, int movi, int n_poses,
+                      hipStream_t st) {
+    const VGeom& v = a.v;
+    const int W = 2 * a.r + 1;
+    const int nb = sep_blocks(v, a.r, grid_ndim);
+    if (nb == 0 || n_poses <= 0) return hipSuccess;
+    const long long V = (long long)v.X * v.Y * v.Z;
+    const int nc = movi ? 7 : 10;
+    for (int pose = 0; pose < n_poses; ++pose) {
+        FusedArgs b = a;
+        b.mats = a.mats + (size_t)pose * a.mats_per_pose * 12;
+        // the moving image on the grid; its |Sobel| when it is the high-resolution image
+        hipError_t e = launch_warp_moving(b, grid_ndim, src_ndim, a.sep_img, st);
+        if (e != hipSuccess) return e;
+        if (movi && (e = launch_sobel(a.sep_img, v, grid_ndim, a.sep_grad, st)) != hipSuccess) return e;
+        const float* I = movi ? a.sep_img : a.stat_img;
+        const float* G = movi ? a.sep_grad : a.stat_grad;
+        const float* U = movi ? a.stat_img : a.sep_img;
+        const int passes = grid_ndim == 3 ? 3 : 2;
+        for (int axis = 0; axis < passes; ++axis) {
+            const long long lines = axis == 0 ? (long long)v.Y * v.Z
+                                  : axis == 1 ? (long long)(v.X - W + 1) * v.Z
+                                              : (long long)(v.X - W + 1) * (v.Y - W + 1);
+            const dim3 grid = grid_for(lines * nc);
+            const double* src = axis == 1 ? a.sep_a : a.sep_b;
+            double* dst = axis == 1 ? a.sep_b : a.sep_a;
+            if (movi)
+                hipLaunchKernelGGL(k_sep_pass<true>, grid, dim3(256), 0, st, I, G, U, src, dst, v, W, axis);
+            else
+                hipLaunchKernelGGL(k_sep_pass<false>, grid, dim3(256), 0, st, I, G, U, src, dst, v, W, axis);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+        const double* S = passes == 3 ? a.sep_a : a.sep_b;
+        double* part = a.partials + (size_t)pose * a.blocks_per_pose * 2;
+        if (movi)
+            hipLaunchKernelGGL(k_sep_combine<true>, dim3(nb), dim3(256), 0, st, S, a.ustat_mean,
+                               a.ustat_cyy, v, a.r, grid_ndim == 3 ? 1 : 0, part, a.wmap, a.mmap);
+        else
+            hipLaunchKernelGGL(k_sep_combine<false>, dim3(nb), dim3(256), 0, st, S, a.ustat_mean,
+                               a.ustat_cyy, v, a.r, grid_ndim == 3 ? 1 : 0, part, a.wmap, a.mmap);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    (void)V;
+    return hipSuccess;
+}
+
+hipError_t launch_reduce(const double* partials, int blocks_per_pose, int n_poses, double* out,
+                         hipStream_t st) {
+    hipLaunchKernelGGL(k_reduce, dim3(n_poses), dim3(256), 0, st, partials, blocks_per_pose, out);
+    return hipGetLastError();
+}
+
 
 hipError_t launch_warp_static(const void* fixed, int fixed_f16, const SGeom& fg, const VGeom& v,
                               int grid_ndim, int src_ndim, float* out, hipStream_t st) {

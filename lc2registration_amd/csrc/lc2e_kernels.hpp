@@ -19,6 +19,7 @@ constexpr int kMaxR3 = 4;      // largest patch radius of the 3D fused kernel
 constexpr int kZBlock = 8;     // 3D partial sums per block of 8 output planes; z-chunks are multiples
 constexpr int kXcds = 8;       // MI355X: 8 XCDs, each with its own L2, 32 CUs each
 constexpr int kMaxR2 = 10;     // largest patch radius of the 2D fused kernel
+constexpr int kMaxRSep = 64;   // largest patch radius of the separable path (larger radii)
 constexpr int kFrameRun = 16;  // frames per 2D work-group (frame stacks)
 
 struct VGeom {  // virtual grid
@@ -67,6 +68,12 @@ struct FusedArgs {
     int mats_per_pose;
     int src_f16;             // moving image stored as half
     int npose;               // poses in this launch (set by launch_fused)
+    // separable path (patch radii beyond the fused kernels): the moving image and its
+    // |Sobel| on the grid (V floats each) and two [channels][V] f64 box-sum buffers
+    float* sep_img;
+    float* sep_grad;
+    double* sep_a;
+    double* sep_b;
 };
 
 // Number of work-groups one pose needs (0 if the grid has no valid patch centre).
@@ -83,6 +90,13 @@ hipError_t fused_resident_per_cu(const FusedArgs& a, int grid_ndim, int src_ndim
                                  int* per_cu);
 hipError_t launch_reduce(const double* partials, int blocks_per_pose, int n_poses, double* out,
                          hipStream_t st);
+// Separable path for radii beyond the fused kernels (3D r > kMaxR3, 2D r > kMaxR2): per
+// pose, warp (+ |Sobel|), box sums along each axis through sep_a / sep_b, combine; one
+// partial per 256 output voxels (sep_blocks per pose, fixed positions).
+int sep_blocks(const VGeom& v, int r, int grid_ndim);
+hipError_t launch_sep(const FusedArgs& a, int grid_ndim, int src_ndim, int movi, int n_poses,
+                      hipStream_t st);
+constexpr int kSepChannels(bool movi) { return movi ? 7 : 10; }
 // Fixed image -> virtual grid at identity (frames of a 2D stack one by one).
 hipError_t launch_warp_static(const void* fixed, int fixed_f16, const SGeom& fg, const VGeom& v,
                               int grid_ndim, int src_ndim, float* out, hipStream_t st);
