@@ -33,6 +33,14 @@ int lc2o_num_threads(void) {
 #endif
 }
 
+/* Threads of the parallel loops that follow (n >= 1); returns the count now in force. */
+int lc2o_set_num_threads(int n) {
+#ifdef _OPENMP
+    if (n >= 1) omp_set_num_threads(n);
+#endif
+    return lc2o_num_threads();
+}
+
 /* ------------------------------------------------------------------------ */
 /* Geometry                                                                  */
 /* ------------------------------------------------------------------------ */
@@ -445,11 +453,23 @@ int lc2o_lc2_exact(const float* U, const float* I, const float* G, const int dim
     if (zhi > outer) zhi = outer;
     if (w_map) memset(w_map, 0, sizeof(double) * XY * Z);
     if (m_map) memset(m_map, 0, sizeof(double) * XY * Z);
-    double sm = 0.0, sw = 0.0;
-    long long nv = 0;
+    /* per-slab partial sums, added in slab order afterwards: the result does not depend on
+     * the thread count or schedule */
+    const int nslab = zhi > zlo ? zhi - zlo : 0;
+    double* psm = (double*)calloc(nslab > 0 ? (size_t)nslab : 1, sizeof(double));
+    double* psw = (double*)calloc(nslab > 0 ? (size_t)nslab : 1, sizeof(double));
+    long long* pnv = (long long*)calloc(nslab > 0 ? (size_t)nslab : 1, sizeof(long long));
+    if (!psm || !psw || !pnv) {
+        free(psm);
+        free(psw);
+        free(pnv);
+        return -1;
+    }
     const int rz = ndim == 3 ? r : 0;
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : sm, sw, nv)
+#pragma omp parallel for schedule(dynamic, 1)
     for (int o = zlo; o < zhi; ++o) {
+        double sm = 0.0, sw = 0.0;
+        long long nv = 0;
         int z0 = ndim == 3 ? o : 0;
         int ylo = ndim == 3 ? 0 : o, yhi = ndim == 3 ? Y : o + 1;
         for (int y = ylo; y < yhi; ++y)
@@ -504,7 +524,20 @@ int lc2o_lc2_exact(const float* U, const float* I, const float* G, const int dim
                 sw += w;
                 nv += 1;
             }
+        psm[o - zlo] = sm;
+        psw[o - zlo] = sw;
+        pnv[o - zlo] = nv;
     }
+    double sm = 0.0, sw = 0.0;
+    long long nv = 0;
+    for (int k = 0; k < nslab; ++k) {
+        sm += psm[k];
+        sw += psw[k];
+        nv += pnv[k];
+    }
+    free(psm);
+    free(psw);
+    free(pnv);
     out->sum_measure = sm;
     out->sum_w = sw;
     out->n_valid = nv;

@@ -97,6 +97,8 @@ double lc2o_nmi(const float* U, const float* I, long long n, int bins_u, int bin
                 int fp32_entropy, int* hist_u, int* hist_i, int* hist_joint);
 
 int lc2o_num_threads(void);
+/* Threads of the parallel loops that follow (n >= 1); returns the count in force. */
+int lc2o_set_num_threads(int n);
 
 #ifdef __cplusplus
 }

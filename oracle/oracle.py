@@ -64,6 +64,8 @@ def lib():
         L.lc2o_eval.argtypes = [fp, gp, fp, gp, gp, C.c_int, C.c_int, C.c_int, C.c_int, fp,
                                 C.c_int, C.c_int, C.POINTER(Result)]
         L.lc2o_num_threads.restype = C.c_int
+        L.lc2o_set_num_threads.argtypes = [C.c_int]
+        L.lc2o_set_num_threads.restype = C.c_int
         L.lc2o_nmi.argtypes = [fp, fp, C.c_longlong, C.c_int, C.c_int, C.c_int, ip, ip, ip]
         L.lc2o_nmi.restype = C.c_double
         _lib = L
@@ -240,3 +242,8 @@ def nmi(U, I, bins_u: int = 256, bins_i: int = 256, fp32: bool = True, hists: bo
 
 def num_threads() -> int:
     return lib().lc2o_num_threads()
+
+
+def set_num_threads(n: int) -> int:
+    """OpenMP threads of the oracle's loops from now on; returns the count in force."""
+    return lib().lc2o_set_num_threads(n)

@@ -3,7 +3,7 @@ projection, and the product's pose -> matrix / MovingToFixed geometry.
 
   * Plateau label maps (with and without noise), linear ramps (constant |Sobel|) and
     exponential ramps, whose |Sobel| is collinear with I (positively for e^{kx}, negatively
-    for c - e^{kx}), through the reference's LC2 class interface (LC2_3D_class /
+    for -e^{kx}), through the reference's LC2 class interface (LC2_3D_class /
     LC2_2D_class via lc2reg_lc2_class_metric): LC2 and weighted map against the oracle.
   * lc2e_pose_to_matrix on a device pair equals the host geometry (pinned against the
     oracle and closed forms in test_geometry_product.py) and the oracle, bit for bit. (The
@@ -65,11 +65,22 @@ def _images(kind, ndim):
     elif kind == "ramp":
         I = (3.0 * x - 2.0 * g[-2] + (0.5 * g[0] if ndim == 3 else 0.0)).astype(np.float32)
     elif kind == "exp_pos":  # |Sobel| of e^{kx} is c e^{kx}: collinear with I
-        I = np.exp(0.05 * x).astype(np.float32)
-    else:  # exp_neg: I = 50 - e^{kx}, |Sobel| = c e^{kx} = c (50 - I)
-        I = (50.0 - np.exp(0.05 * x)).astype(np.float32)
-    # U follows I and |grad I| loosely so the fit has something to explain
-    U = (U + 0.05 * np.abs(I)).astype(np.float32)
+        I = np.exp(0.25 * x).astype(np.float32)
+    else:  # exp_neg: I = -e^{kx}, |Sobel| = c e^{kx} = -c I
+        I = (-np.exp(0.25 * x)).astype(np.float32)
+    # (k = 0.25: a patch's spread of I is a large fraction of its mean, so the f32 rounding
+    # of I and G leaves 1 - rho^2 near 1e-13, far below the 1e-10 collinearity threshold)
+    if not kind.startswith("exp"):
+        # U follows I and |grad I| loosely so the fit has something to explain
+        U = (U + 0.05 * np.abs(I)).astype(np.float32)
+    # U = 0 on the faces: patches that reach the zero-padded |Sobel| of the border rows are
+    # excluded, so every patch is clearly inside or clearly outside a rank class (the rule
+    # is discontinuous at its threshold, where rounding may decide either way)
+    for ax in range(U.ndim):
+        idx = [slice(None)] * U.ndim
+        for e in (0, -1):
+            idx[ax] = e
+            U[tuple(idx)] = 0.0
     return U, I
 
 
