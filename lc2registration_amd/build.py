@@ -27,7 +27,7 @@ DEVICE_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-co
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
 
 DEVICE_SOURCES = ["lc2e_kernels.hip"]
-HOST_SOURCES = ["lc2e_engine.cpp", "lc2e_serve.cpp", "facade.cpp", "yaml_lite.cpp", "optim.cpp", "bobyqa.cpp", "image_io.cpp",
+HOST_SOURCES = ["lc2e_engine.cpp", "lc2e_serve.cpp", "lc2e_sessions.cpp", "facade.cpp", "yaml_lite.cpp", "optim.cpp", "bobyqa.cpp", "igtl.cpp", "image_io.cpp",
                 "lc2reg_capi.cpp"]
 CLI_SOURCES = ["main_registration.cpp"]
 

@@ -59,9 +59,10 @@ inline void mat3_mul(const double a[9], const double b[9], double c[9]) {
 // VirtualToMoving M(x) = A (x - C + shift) + C + T, A = Rz Ry Rx Sh diag(S) (3D) or
 // R Sh diag(S) (2D), degrees. The reference casts its double parameters to float first
 // (optimizers.cu:677-694); the matrix is composed in FP64 and rounded once.
-inline void affine_matrix_3d(const float T[3], const float Rdeg[3], const float Sc[3],
-                             const float Sh[3], const float C[3], const float shift[3],
-                             float out12[12]) {
+// The FP64 linear part A (3x3 row-major) and translation t of that matrix, before rounding.
+inline void affine_3d_fp64(const float T[3], const float Rdeg[3], const float Sc[3],
+                           const float Sh[3], const float C[3], const float shift[3],
+                           double A[9], double t[3]) {
     const double d2r = 3.14159265358979323846 / 180.0;
     const double cx = std::cos(Rdeg[0] * d2r), sx = std::sin(Rdeg[0] * d2r);
     const double cy = std::cos(Rdeg[1] * d2r), sy = std::sin(Rdeg[1] * d2r);
@@ -71,16 +72,25 @@ inline void affine_matrix_3d(const float T[3], const float Rdeg[3], const float 
     const double rz[9] = {cz, -sz, 0, sz, cz, 0, 0, 0, 1};
     const double sh[9] = {1, Sh[0], Sh[1], 0, 1, Sh[2], 0, 0, 1};
     const double sc[9] = {Sc[0], 0, 0, 0, Sc[1], 0, 0, 0, Sc[2]};
-    double t0[9], t1[9], t2[9], A[9];
+    double t0[9], t1[9], t2[9];
     mat3_mul(rz, ry, t0);
     mat3_mul(t0, rx, t1);
     mat3_mul(t1, sh, t2);
     mat3_mul(t2, sc, A);
     for (int i = 0; i < 3; ++i) {
-        double t = (double)C[i] + (double)T[i];
-        for (int k = 0; k < 3; ++k) t += A[i * 3 + k] * ((double)shift[k] - (double)C[k]);
+        t[i] = (double)C[i] + (double)T[i];
+        for (int k = 0; k < 3; ++k) t[i] += A[i * 3 + k] * ((double)shift[k] - (double)C[k]);
+    }
+}
+
+inline void affine_matrix_3d(const float T[3], const float Rdeg[3], const float Sc[3],
+                             const float Sh[3], const float C[3], const float shift[3],
+                             float out12[12]) {
+    double A[9], t[3];
+    affine_3d_fp64(T, Rdeg, Sc, Sh, C, shift, A, t);
+    for (int i = 0; i < 3; ++i) {
         for (int k = 0; k < 3; ++k) out12[i * 4 + k] = (float)A[i * 3 + k];
-        out12[i * 4 + 3] = (float)t;
+        out12[i * 4 + 3] = (float)t[i];
     }
 }
 

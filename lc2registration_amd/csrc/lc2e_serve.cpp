@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -30,6 +31,8 @@ int engine_alloc(lc2e_engine* e, void** p, size_t bytes);
 void engine_release(lc2e_engine* e, void* p, size_t bytes);
 int engine_device(lc2e_engine* e);
 double launch_share(lc2e_pair* p, int n);
+unsigned long long pair_generation(lc2e_pair* p);
+int pair_prepare(lc2e_pair* p, int n);
 }  // namespace lc2e
 
 namespace {
@@ -55,6 +58,22 @@ int pair_mats_per_pose(lc2e_pair* p) {
     return info.per_frame_matrices ? info.n_frames : 1;
 }
 
+// A launch captured as a HIP graph on a slot: upload, kernels, download for one pair and
+// batch size, valid while the pair's launch state keeps its generation.
+struct SlotGraph {
+    unsigned long long gen = 0;
+    int uses = 0;  // launches seen for this key (the first one runs eagerly)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+void destroy_graph(SlotGraph& g) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g.exec = nullptr;
+    g.graph = nullptr;
+}
+
 // A launch slot: a stream, a completion event and staging buffers sized on demand.
 struct Slot {
     hipStream_t stream = nullptr;
@@ -70,6 +89,7 @@ struct Slot {
     std::vector<RequestPtr> batch;
     int total = 0;
     double share = 0.0;  // of the device (lc2e::launch_share)
+    std::map<std::pair<lc2e_pair*, int>, SlotGraph> graphs;  // by (pair, poses)
 };
 
 void fill_error(Request& r, int rc, const std::string& err) {
@@ -107,6 +127,8 @@ struct lc2s_server {
     // worker thread only
     std::set<lc2e_pair*> in_flight;  // pairs with a launch on the device
     long long extra_launches = 0;    // per-request launches of split merged launches
+    std::atomic<bool> use_graphs{true};        // replay captured launches (lc2s_use_graphs)
+    std::atomic<long long> graph_launches{0};
 
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -129,6 +151,8 @@ struct lc2s_server {
     }
 
     void free_buffers(Slot& s) {
+        for (auto& kv : s.graphs) destroy_graph(kv.second);  // they captured these buffers
+        s.graphs.clear();
         if (s.d_mats) lc2e::engine_release(engine, s.d_mats, s.mats_cap * sizeof(float));
         if (s.d_out) lc2e::engine_release(engine, s.d_out, (size_t)s.out_cap * 3 * sizeof(double));
         if (s.h_mats) (void)hipHostFree(s.h_mats);
@@ -235,7 +259,8 @@ struct lc2s_server {
 
     // Outside the lock: upload, launch and download on the slot's stream, then record its
     // event. The slot's staging and the pair's partial sums are not in use by the device.
-    int enqueue(Slot& s, int n, size_t floats) {
+    // The launch itself: upload, lc2e_eval_batch_async, download on the slot's stream.
+    int enqueue_ops(Slot& s, int n, size_t floats) {
         if (hipMemcpyAsync(s.d_mats, s.h_mats, floats * sizeof(float), hipMemcpyHostToDevice,
                            s.stream) != hipSuccess) {
             lc2e::set_last_error("serving: matrix upload failed");
@@ -244,9 +269,61 @@ struct lc2s_server {
         const int rc = lc2e_eval_batch_async(s.pair, n, s.d_mats, s.d_out, s.stream);
         if (rc < 0) return rc;
         if (hipMemcpyAsync(s.h_out, s.d_out, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost,
-                           s.stream) != hipSuccess ||
-            hipEventRecord(s.done, s.stream) != hipSuccess) {
+                           s.stream) != hipSuccess) {
             lc2e::set_last_error("serving: result download failed");
+            return LC2E_ERR_HIP;
+        }
+        return LC2E_OK;
+    }
+
+    // A (pair, n) launch runs eagerly the first time (which also sizes the pair's buffers
+    // and does one-time kernel set-up), is captured as a HIP graph the second time and
+    // replayed after that, until the pair's launch state changes (generation).
+    int enqueue(Slot& s, int n, size_t floats) {
+        int rc = LC2E_OK;
+        SlotGraph* g = nullptr;
+        if (use_graphs) {
+            if ((rc = lc2e::pair_prepare(s.pair, n)) < 0) return rc;  // no allocation when captured
+            g = &s.graphs[{s.pair, n}];
+            const unsigned long long gen = lc2e::pair_generation(s.pair);
+            if (g->gen != gen) {
+                destroy_graph(*g);
+                g->gen = gen;
+                g->uses = 0;
+            }
+            if (!g->exec && g->uses >= 1) {
+                hipGraph_t graph = nullptr;
+                if (hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                    lc2e::set_last_error("serving: stream capture failed");
+                    return LC2E_ERR_HIP;
+                }
+                rc = enqueue_ops(s, n, floats);
+                hipError_t ce = hipStreamEndCapture(s.stream, &graph);
+                if (rc >= 0 && ce == hipSuccess) ce = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+                if (rc < 0 || ce != hipSuccess) {
+                    if (graph) (void)hipGraphDestroy(graph);
+                    g->exec = nullptr;
+                    if (rc >= 0) {
+                        lc2e::set_last_error("serving: graph capture failed");
+                        rc = LC2E_ERR_HIP;
+                    }
+                    return rc;
+                }
+                g->graph = graph;
+            }
+            ++g->uses;
+        }
+        if (g && g->exec) {
+            if (hipGraphLaunch(g->exec, s.stream) != hipSuccess) {
+                lc2e::set_last_error("serving: graph launch failed");
+                return LC2E_ERR_HIP;
+            }
+            ++graph_launches;
+        } else if ((rc = enqueue_ops(s, n, floats)) < 0) {
+            return rc;
+        }
+        if (hipEventRecord(s.done, s.stream) != hipSuccess) {
+            lc2e::set_last_error("serving: completion event failed");
             return LC2E_ERR_HIP;
         }
         return LC2E_OK;
@@ -485,6 +562,16 @@ int lc2s_stats(lc2s_server* s, lc2s_counters* c) {
     c->poses = s->poses;
     c->queued = (long long)s->queue.size();
     c->max_in_flight = s->busy_max;
+    c->graph_launches = s->graph_launches;
+    return LC2E_OK;
+}
+
+int lc2s_use_graphs(lc2s_server* s, int enable) {
+    if (!s) {
+        lc2e::set_last_error("lc2s_use_graphs: invalid argument");
+        return LC2E_ERR_INVALID;
+    }
+    s->use_graphs = enable != 0;
     return LC2E_OK;
 }
 

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -21,6 +22,7 @@ struct lc2e_engine {
 };
 struct lc2e_pair {
     int id = 0;
+    unsigned long long gen = 1;  // launch-state generation (graphs)
     int mpp = 1;               // matrices per pose
     double share = 0.1;        // device share per pose
     int fail_code = 0;         // < 0: every launch of this pair fails with it
@@ -48,6 +50,20 @@ struct FakeEvent {
 };
 std::vector<FakeEvent*> g_events;
 
+// Stream capture: operations enqueued on a capturing stream are recorded instead of run,
+// and a graph launch runs them again in order.
+struct FakeGraph {
+    std::vector<std::function<void()>> ops;
+};
+std::map<void*, FakeGraph*> g_capture;  // stream -> graph being captured (guarded by g_mu)
+int g_graph_replays = 0;
+
+FakeGraph* capturing(void* stream) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_capture.find(stream);
+    return it == g_capture.end() ? nullptr : it->second;
+}
+
 }  // namespace
 
 extern "C" {
@@ -71,6 +87,7 @@ void fake_config(int query_delay, long long budget_bytes, int budget_poses) {
     g_budget_poses = budget_poses;
     g_log.clear();
     g_in_flight = g_max_in_flight = 0;
+    g_graph_replays = 0;
 }
 int fake_launches() {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -85,6 +102,11 @@ int fake_launch(int k, int* pair, int* n, float* first, int cap) {
     for (int i = 0; i < g_log[k].n && i < cap; ++i) first[i] = g_log[k].first[i];
     return 0;
 }
+int fake_graph_replays() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return g_graph_replays;
+}
+void fake_bump_generation(lc2e_pair* p) { ++p->gen; }
 int fake_max_in_flight() {
     std::lock_guard<std::mutex> lk(g_mu);
     return g_max_in_flight;
@@ -110,7 +132,9 @@ int lc2e_pair_get_info(lc2e_pair* p, lc2e_pair_info* info) {
 
 // Fake results per pose: sum of the pose's matrix floats as Σmeasure, 1 as Σw (0 if any
 // float is NaN), LC2 = Σmeasure / Σw with NaN -> 0.
-int lc2e_eval_batch_async(lc2e_pair* p, int n, const float* d_mats, double* d_out, void*) {
+static void fake_compute(lc2e_pair* p, int n, const float* d_mats, double* d_out);
+
+int lc2e_eval_batch_async(lc2e_pair* p, int n, const float* d_mats, double* d_out, void* stream) {
     if (p->fail_code < 0) {
         g_err = "fake launch failure of pair " + std::to_string(p->id);
         return p->fail_code;
@@ -121,6 +145,18 @@ int lc2e_eval_batch_async(lc2e_pair* p, int n, const float* d_mats, double* d_ou
             g_err = "fake budget";
             return LC2E_ERR_BUDGET;
         }
+    }
+    if (FakeGraph* g = capturing(stream)) {
+        g->ops.push_back([=] { fake_compute(p, n, d_mats, d_out); });
+        return LC2E_OK;
+    }
+    fake_compute(p, n, d_mats, d_out);
+    return LC2E_OK;
+}
+
+static void fake_compute(lc2e_pair* p, int n, const float* d_mats, double* d_out) {
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
         Launch l{p->id, n, {}};
         for (int i = 0; i < n; ++i) l.first.push_back(d_mats[(size_t)i * p->mpp * 12]);
         g_log.push_back(l);
@@ -137,7 +173,6 @@ int lc2e_eval_batch_async(lc2e_pair* p, int n, const float* d_mats, double* d_ou
         d_out[3 * i + 1] = w;
         d_out[3 * i + 2] = std::isfinite(l) ? l : 0.0;
     }
-    return LC2E_OK;
 }
 
 // --- HIP runtime subset ----------------------------------------------------------------
@@ -200,8 +235,42 @@ hipError_t hipEventQuery(hipEvent_t e) {
     }
     return hipSuccess;
 }
-hipError_t hipMemcpyAsync(void* dst, const void* src, size_t bytes, hipMemcpyKind, hipStream_t) {
+hipError_t hipMemcpyAsync(void* dst, const void* src, size_t bytes, hipMemcpyKind, hipStream_t s) {
+    if (FakeGraph* g = capturing(s)) {
+        g->ops.push_back([=] { std::memcpy(dst, src, bytes); });
+        return hipSuccess;
+    }
     std::memcpy(dst, src, bytes);
+    return hipSuccess;
+}
+hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_capture.count(s)) return hipErrorInvalidValue;
+    g_capture[s] = new FakeGraph();
+    return hipSuccess;
+}
+hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t* graph) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_capture.find(s);
+    if (it == g_capture.end()) return hipErrorInvalidValue;
+    *graph = reinterpret_cast<hipGraph_t>(it->second);
+    g_capture.erase(it);
+    return hipSuccess;
+}
+hipError_t hipGraphInstantiate(hipGraphExec_t* exec, hipGraph_t graph, hipGraphNode_t*, char*, size_t) {
+    *exec = reinterpret_cast<hipGraphExec_t>(graph);
+    return hipSuccess;
+}
+hipError_t hipGraphLaunch(hipGraphExec_t exec, hipStream_t) {
+    auto* g = reinterpret_cast<FakeGraph*>(exec);
+    for (auto& op : g->ops) op();
+    std::lock_guard<std::mutex> lk(g_mu);
+    ++g_graph_replays;
+    return hipSuccess;
+}
+hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
+hipError_t hipGraphDestroy(hipGraph_t graph) {
+    delete reinterpret_cast<FakeGraph*>(graph);
     return hipSuccess;
 }
 hipError_t hipHostMalloc(void** p, size_t bytes, unsigned int) {
@@ -244,4 +313,6 @@ void engine_release(lc2e_engine*, void* p, size_t bytes) {
 }
 int engine_device(lc2e_engine*) { return 0; }
 double launch_share(lc2e_pair* p, int n) { return p->share * n; }
+unsigned long long pair_generation(lc2e_pair* p) { return p->gen; }
+int pair_prepare(lc2e_pair*, int) { return LC2E_OK; }
 }  // namespace lc2e

@@ -22,6 +22,8 @@ void fake_config(int query_delay, long long budget_bytes, int budget_poses);
 int fake_launches();
 int fake_launch(int k, int* pair, int* n, float* first, int cap);
 int fake_max_in_flight();
+int fake_graph_replays();
+void fake_bump_generation(lc2e_pair* p);
 long long fake_used();
 void fake_fail_host_malloc(int after);
 }
@@ -272,6 +274,41 @@ void scenario_bad_arguments() {
 
 }  // namespace
 
+// Graph replay: a (pair, poses) launch runs eagerly once, is captured the second time and
+// replayed after that; results stay those of the eager launch; a new generation of the pair
+// (buffers reallocated) is captured again; with graphs off every launch is eager.
+void scenario_graph_replay() {
+    fake_config(2, 0, 0);
+    lc2e_engine* e = fake_engine();
+    lc2e_pair* a = fake_pair(1, 1, 0.1, 0);
+    lc2s_server* s = nullptr;
+    CHECK(lc2s_create(e, 16, 1, 0.0, &s) == LC2E_OK);
+    auto run = [&](int k) {
+        std::vector<float> m = poses(3, 1, 10.0f * k);
+        long long t = 0;
+        CHECK(lc2s_submit(s, a, 3, m.data(), &t) == LC2E_OK);
+        std::vector<double> l(3, -1.0);
+        CHECK(lc2s_wait(s, t, -1.0, l.data(), nullptr, nullptr) == LC2E_OK);
+        for (int i = 0; i < 3; ++i) CHECK(l[i] == expect_lc2(m.data() + 12 * i, 1));
+    };
+    for (int k = 0; k < 6; ++k) run(k);
+    lc2s_counters c{};
+    CHECK(lc2s_stats(s, &c) == LC2E_OK);
+    CHECK(c.graph_launches == 5 && fake_graph_replays() == 5 && fake_launches() == 6);
+    fake_bump_generation(a);  // the pair's launch state changed: capture again
+    for (int k = 6; k < 9; ++k) run(k);
+    CHECK(lc2s_stats(s, &c) == LC2E_OK);
+    CHECK(c.graph_launches == 7 && fake_launches() == 9);
+    CHECK(lc2s_use_graphs(s, 0) == LC2E_OK);
+    for (int k = 9; k < 12; ++k) run(k);
+    CHECK(lc2s_stats(s, &c) == LC2E_OK);
+    CHECK(c.graph_launches == 7 && fake_launches() == 12);
+    lc2s_destroy(s);
+    fake_free_pair(a);
+    fake_free_engine(e);
+    std::printf("graph_replay ok: %lld of 12 launches replayed\n", c.graph_launches);
+}
+
 int main() {
     scenario_merge_and_equality();
     scenario_errors_and_budget();
@@ -279,6 +316,7 @@ int main() {
     scenario_destroy_with_work();
     scenario_staging_failure();
     scenario_bad_arguments();
+    scenario_graph_replay();
     std::printf("all serving scenarios passed\n");
     return 0;
 }

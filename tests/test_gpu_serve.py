@@ -75,6 +75,35 @@ def test_mixed_requests_equal_alone_and_are_merged(engine):
         mv.close()
 
 
+@pytest.mark.parametrize("graphs", [True, False])
+def test_graph_replay_equals_eager_evaluation(engine, graphs):
+    """Same-size requests of a pair one after another: from the second launch on each
+    slot replays a captured HIP graph; every result equals lc2e_eval_batch bit for bit."""
+    pairs = _pairs(engine)
+    rng = np.random.default_rng(11)
+    work = []
+    for i, (w, p, _, _) in enumerate(pairs):
+        for _ in range(5):
+            mats = np.stack([p.pose_to_matrix(synth.perturb(w.true_pose, rng, w.ndim))
+                             for _ in range(4)])
+            work.append((i, mats, p.eval_batch(mats)))
+    srv = S.Server(engine, max_poses=8, slots=1)
+    srv.use_graphs(graphs)
+    for i, mats, alone in work:
+        r = srv.wait(srv.submit(pairs[i][1], mats))
+        assert r.status == alone.status
+        assert np.array_equal(r.lc2, alone.lc2) and np.array_equal(r.sum_w, alone.sum_w)
+    st = srv.stats()
+    srv.close()
+    print("serving counters", st)
+    # per pair: launch 1 eager, launch 2 captured and replayed, then replays
+    assert st["graph_launches"] == (8 if graphs else 0)
+    for _, p, fv, mv in pairs:
+        p.close()
+        fv.close()
+        mv.close()
+
+
 def test_time_out_and_unknown_ticket(engine):
     """A request held by the batching window is not done after a zero time-out (the
     time-out code, the request stays queued) and completes on the next wait; an unknown
