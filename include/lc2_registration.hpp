@@ -23,6 +23,9 @@
 //    NLopt variants) and Powell's BOBYQA; DIRECT iterations and BOBYQA's initial
 //    interpolation set are evaluated in batches (one fused launch each).
 //  * Images come from raw files or NRRD (the reference reads through ITK).
+//  * Interpolator3D(yaml, useIGTL = true) opens the OpenIGTLink server and waits for its
+//    client as the reference does; the pose of every evaluation (the optimizers' batches
+//    included) is streamed, in evaluation order (lc2igtl.h).
 #pragma once
 
 #include <hip/hip_vector_types.h>
@@ -34,6 +37,7 @@
 #include <vector>
 
 #include "lc2e.h"
+#include "lc2igtl.h"
 
 // ---- RegistrationParametersConfig.h (programmatic configuration) ----
 struct regIntPoints3D { int x, y, z; };
@@ -153,6 +157,10 @@ namespace lc2reg {
 
 // Shared engine state of one process (one device); created on first use.
 lc2e_engine* shared_engine();
+// TCP port of the OpenIGTLink server of Interpolator3D(yaml, useIGTL = true); the
+// reference's is 23333 (interpolator.cu:366).
+void set_igtl_port(int port);
+int igtl_port();
 
 // Geometry + data of one image as resident engine volume.
 struct LoadedImage {
@@ -204,6 +212,10 @@ protected:
     lc2e_pair_info info_{};
     float current_[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
     int counter = 0;
+    // OpenIGTLink pose stream (3D with useIGTL): every evaluated pose is sent as TRANSFORM
+    lc2igtl_server* igtl_ = nullptr;
+    void start_igtl();
+    void stream_pose(const float m12[12]);
 };
 
 }  // namespace lc2reg

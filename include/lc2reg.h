@@ -40,6 +40,14 @@ int lc2reg_moving_to_fixed(const lc2e_pair_info* info, const double* params, int
 int lc2reg_register_yaml(const char* mode, const char* yaml_path, int write_images,
                          double* out_params, double* out_lc2, int* out_nevals, float* out_matrix);
 
+/* lc2reg_register_yaml with the reference CLI's OpenIGTLink pose stream (main.cu:761-769,
+ * interpolator.cu:359-388, :545-573), 3D only: igtl_port > 0 opens the server on that port
+ * and waits for a client before the registration; every evaluated pose is then sent as a
+ * TRANSFORM (lc2igtl.h). igtl_port 0: no server (lc2reg_register_yaml). */
+int lc2reg_register_yaml_igtl(const char* mode, const char* yaml_path, int write_images,
+                              int igtl_port, double* out_params, double* out_lc2, int* out_nevals,
+                              float* out_matrix);
+
 /* LC2 of one parameter vector for a YAML-configured pair (Interpolator*::GenerateMappers +
  * Interpolate + GetSimilarityMeasure with the YAML's PatchSize). */
 int lc2reg_evaluate_yaml(const char* mode, const char* yaml_path, const double* params, int n_params,

@@ -75,7 +75,36 @@ lc2e_engine* shared_engine() {
 // Interpolators
 // ---------------------------------------------------------------------------
 
+namespace {
+int g_igtl_port = 23333;
+}  // namespace
+
+void set_igtl_port(int port) { g_igtl_port = port; }
+int igtl_port() { return g_igtl_port; }
+
+// interpolator.cu:359-388: server on the port, then wait (polling every 200 ms) for the
+// client before anything else happens.
+void InterpolatorBase::start_igtl() {
+    if (lc2igtl_server_create(g_igtl_port, 0, &igtl_) < 0)
+        throw std::runtime_error(std::string("Cannot create a server socket: ") + lc2igtl_last_error());
+    std::cout << "Registration Server: server created, localhost:" << lc2igtl_port(igtl_) << std::endl;
+    if (lc2igtl_wait_client(igtl_, -1.0) != 1)
+        throw std::runtime_error(std::string("OpenIGTLink: ") + lc2igtl_last_error());
+    std::cout << "Registration Server: client connected" << std::endl;
+}
+
+// interpolator.cu:545-573: the current transform (GetMyselfInversed of VirtualToMoving) as
+// TRANSFORM, device "RegistrationServer", header version 2.
+void InterpolatorBase::stream_pose(const float m12[12]) {
+    if (!igtl_) return;
+    float inv[16];
+    if (!lc2e::invert_affine(m12, inv)) return;
+    if (lc2igtl_send_transform(igtl_, "RegistrationServer", inv) < 0)
+        std::cerr << "OpenIGTLink: " << lc2igtl_last_error() << std::endl;
+}
+
 InterpolatorBase::~InterpolatorBase() {
+    if (igtl_) lc2igtl_server_destroy(igtl_);
     if (pair_) lc2e_pair_destroy(pair_);
     if (fixed_.vol) lc2e_volume_destroy(fixed_.vol);
     if (moving_.vol) lc2e_volume_destroy(moving_.vol);
@@ -167,6 +196,8 @@ void InterpolatorBase::InitiateRegistrationHandle() {
                            patch_size_ > 0 ? patch_size_ : 1, grid_ndim_, &pair_),
           "pair create");
     check(lc2e_pair_get_info(pair_, &info_), "pair info");
+    // every batch size the optimizers use is captured once as a HIP graph and replayed
+    check(lc2e_pair_use_graph(pair_, 1), "graph mode");
 }
 
 void InterpolatorBase::set_patch_size(int patchSize) {
@@ -179,6 +210,7 @@ void InterpolatorBase::set_patch_size(int patchSize) {
 void InterpolatorBase::set_pose(const double* params, int n) {
     if (!pair_) InitiateRegistrationHandle();
     check(lc2e_pose_to_matrix(pair_, params, n, current_), "pose");
+    stream_pose(current_);
     ++counter;
 }
 
@@ -196,9 +228,11 @@ int InterpolatorBase::EvaluateBatch(const std::vector<std::vector<double>>& para
                                     std::vector<double>& lc2) {
     if (!pair_) InitiateRegistrationHandle();
     std::vector<float> mats(params.size() * 12);
-    for (size_t i = 0; i < params.size(); ++i)
+    for (size_t i = 0; i < params.size(); ++i) {
         check(lc2e_pose_to_matrix(pair_, params[i].data(), (int)params[i].size(), &mats[i * 12]),
               "pose");
+        stream_pose(&mats[i * 12]);  // the reference sends at every GenerateMappers
+    }
     lc2.assign(params.size(), 0.0);
     const int rc = patch_size_ == 0
                        ? lc2e_eval_mi_batch(pair_, (int)params.size(), mats.data(), 256, 256, lc2.data())
@@ -254,7 +288,10 @@ void InterpolatorBase::write_out(const float* matrix, int mdim) {
 }  // namespace lc2reg
 
 // ---- Interpolator3D ----
-Interpolator3D::Interpolator3D(const std::string yaml_filePath, bool) { load_from_yaml(yaml_filePath, 3, false); }
+Interpolator3D::Interpolator3D(const std::string yaml_filePath, bool useIGTL) {
+    load_from_yaml(yaml_filePath, 3, false);
+    if (useIGTL) start_igtl();
+}
 
 Interpolator3D::Interpolator3D(const LC2Configuration3D* c) {
     if (!c || !c->IsConfigured) throw std::runtime_error("LC2Configuration3D is not configured");

@@ -5,6 +5,7 @@
 #include <exception>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "lc2_registration.hpp"
@@ -23,8 +24,11 @@ int fail(const std::exception& e) {
 
 template <class Interp, class Opt>
 int run(const char* yaml, int write_images, int nparams, int mdim, double* out_params,
-        double* out_lc2, int* out_nevals, float* out_matrix) {
-    Interp interp{std::string(yaml)};
+        double* out_lc2, int* out_nevals, float* out_matrix, bool igtl = false) {
+    Interp interp = [&]() -> Interp {
+        if constexpr (std::is_same<Interp, Interpolator3D>::value) return Interp{std::string(yaml), igtl};
+        else return Interp{std::string(yaml)};
+    }();
     interp.InitiateMappers();
     interp.InitiateRegistrationHandle();
     Opt opt{std::string(yaml)};
@@ -128,11 +132,19 @@ int lc2reg_lc2_class_metric(int x, int y, int z, const float* us, const float* h
 
 int lc2reg_register_yaml(const char* mode, const char* yaml_path, int write_images,
                          double* out_params, double* out_lc2, int* out_nevals, float* out_matrix) {
+    return lc2reg_register_yaml_igtl(mode, yaml_path, write_images, 0, out_params, out_lc2,
+                                     out_nevals, out_matrix);
+}
+
+int lc2reg_register_yaml_igtl(const char* mode, const char* yaml_path, int write_images,
+                              int igtl_port, double* out_params, double* out_lc2, int* out_nevals,
+                              float* out_matrix) {
     try {
         const std::string m = mode ? mode : "";
+        if (igtl_port > 0) lc2reg::set_igtl_port(igtl_port);
         if (m == "3D")
             return run<Interpolator3D, Optimizers3D>(yaml_path, write_images, 12, 4, out_params,
-                                                     out_lc2, out_nevals, out_matrix);
+                                                     out_lc2, out_nevals, out_matrix, igtl_port > 0);
         if (m == "2D")
             return run<Interpolator2D, Optimizers2D>(yaml_path, write_images, 6, 3, out_params,
                                                      out_lc2, out_nevals, out_matrix);
@@ -152,7 +164,9 @@ int lc2reg_evaluate_yaml(const char* mode, const char* yaml_path, const double* 
     try {
         const std::string m = mode ? mode : "";
         std::unique_ptr<lc2reg::InterpolatorBase> ip;
-        if (m == "3D") ip.reset(new Interpolator3D(std::string(yaml_path)));
+        // one evaluation streams nothing: no OpenIGTLink server (the constructor's default
+        // opens one and waits for its client)
+        if (m == "3D") ip.reset(new Interpolator3D(std::string(yaml_path), false));
         else if (m == "2D") ip.reset(new Interpolator2D(std::string(yaml_path)));
         else if (m == "2DTo3D") ip.reset(new Interpolator2DTo3D(std::string(yaml_path)));
         else {
