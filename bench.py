@@ -28,6 +28,7 @@ FP64_PEAK = 78.6e12
 ALG_BYTES_PER_VOXEL = 1.0 + 0.5
 ALG_F32_PER_VOXEL = 116.0   # warp 40, Sobel 27, products 5, window sums 42, reduction 2
 ALG_F64_PER_VOXEL = 45.0    # centred moments, 2x2 solve, variance/measure, thresholds
+ALG_F32_SWEEP = 89.0        # 2D frames from 3D: warp 40, Sobel 14, products 5, sums 28, reduction 2
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "k_fused3d_r3_affine3d_us192_traffic.json")
 SIMDS, CLOCK_HZ = 256 * 4, 2.4e9  # MI355X: 256 CUs of 4 SIMDs, peak engine clock
 
@@ -36,44 +37,211 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(w, M, budget_s):
-    """The reference's algorithm (FP32 CGLS per voxel, restated in the oracle) on the
-    host cores: full warps + Sobel, CGLS on a slab of output planes, scaled to the grid."""
-    from oracle import oracle as O
+def _f32_equiv(units, f32=ALG_F32_PER_VOXEL, f64=ALG_F64_PER_VOXEL):
+    """Algorithmic flops of `units` voxel-poses in FP32 equivalents: an FP64 flop takes
+    two FP32 issue slots (78.6 vs 157.3 TFLOP/s), so achieved/157.3 = t_VALU/t."""
+    return units * (f32 + f64 * FP32_PEAK / FP64_PEAK)
+
+
+def cpu_model():
+    """The host CPU's model name (as lscpu reports it) and the cores this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0)), os.cpu_count()
+
+
+def _oracle_grids(O, w, ndim=3):
     f, m = w.fixed, w.moving
     fg, mg = O.geom(f.dims, f.spacing, f.origin), O.geom(m.dims, m.spacing, m.origin)
-    vd = O.virtual_dims(fg, w.resampled_spacing, 3)
-    vg = O.geom(vd, w.resampled_spacing, f.origin)
+    vd = O.virtual_dims(fg, w.resampled_spacing, ndim)
+    return fg, mg, O.geom(vd, w.resampled_spacing, f.origin), vd
+
+
+def cpu_baseline(w, M, budget_s, checks=()):
+    """The CPU leg. (1) The reference's algorithm (FP32 CGLS per voxel, restated in the
+    oracle) on the host cores: full warps + Sobel, CGLS on a slab of output planes, scaled
+    to the grid; with all OpenMP threads, then with one. (2) The FP64 oracle as the checker
+    of the GPU figures in `checks`: each is (name, workload, kind, matrices, GPU LC2s,
+    moving data as the GPU stored it); |dLC2| is reported against the FP64 oracle on the
+    workload's own data and, for rounded storage, on the rounded data the GPU read."""
+    from oracle import oracle as O
+    f, m = w.fixed, w.moving
+    fg, mg, vg, vd = _oracle_grids(O, w)
     r = w.patch_radius
-    t0 = time.perf_counter()
-    U = O.warp_3d(f.data.astype(np.float32), fg, vg, [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0])
-    I = O.warp_3d(m.data, mg, vg, M)
-    G = O.sobel(I, "ref")
-    t_ws = time.perf_counter() - t0
     planes = vd[2] - 2 * r
     zc = vd[2] // 2
+    all_threads = O.num_threads()
 
-    def timed(fn, k):
-        t = time.perf_counter()
-        fn(zc - k // 2, zc - k // 2 + k)
-        return time.perf_counter() - t
+    def sample(threads, names, budget):
+        O.set_num_threads(threads)
+        t0 = time.perf_counter()
+        U = O.warp_3d(f.data.astype(np.float32), fg, vg, [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0])
+        I = O.warp_3d(m.data, mg, vg, M)
+        G = O.sobel(I, "ref")
+        t_ws = time.perf_counter() - t0
 
+        def timed(fn, k):
+            t = time.perf_counter()
+            fn(zc - k // 2, zc - k // 2 + k)
+            return time.perf_counter() - t
+
+        fns = {"cg_fp32": lambda a, b: O.lc2_cg(U, I, G, r, zlo=a, zhi=b),
+               "exact_fp64": lambda a, b: O.lc2_exact(U, I, G, r, zlo=a, zhi=b)}
+        out = {}
+        for name in names:
+            t1 = timed(fns[name], 1)
+            k = int(max(1, min(planes, budget / max(t1, 1e-6))))
+            tk = timed(fns[name], k) if k > 1 else t1
+            t_eval = t_ws + tk * planes / k
+            out[name] = (1.0 / t_eval, k, t_eval, t_ws)
+            log(f"cpu {name}, {threads} thread(s): warps+Sobel {t_ws:.2f}s, {k} of {planes} "
+                f"planes in {tk:.2f}s -> {t_eval:.2f} s/eval")
+        return out
+
+    multi = sample(all_threads, ("cg_fp32", "exact_fp64"), budget_s)
+    one = sample(1, ("cg_fp32",), budget_s)
+    O.set_num_threads(all_threads)
+    model, usable, logical = cpu_model()
+    v, k, t_eval, _ = multi["cg_fp32"]
+    v1, k1, t1_eval, t1_ws = one["cg_fp32"]
+    out = {"value": v, "unit": "LC2 evals/s", "cores": all_threads, "kind": "port",
+           "sample": (f"reference algorithm (FP32 CGLS, <=20 it) restated in oracle/, "
+                      f"{all_threads} OpenMP threads: full warps + Sobel, CGLS on {k} of {planes} "
+                      f"output planes, scaled; {t_eval:.2f} s per evaluation"),
+           "cpu_model": model, "cpus_usable": usable, "cpus_logical": logical,
+           "one_thread": {"value": v1, "unit": "LC2 evals/s", "cores": 1,
+                          "sample": f"warps + Sobel in full ({t1_ws:.2f} s), CGLS on {k1} of "
+                                    f"{planes} output planes, scaled; {t1_eval:.1f} s per "
+                                    f"evaluation"},
+           "exact_fp64_evals_per_s": multi["exact_fp64"][0]}
+    if checks:
+        out["fp64_oracle_check"] = fp64_oracle_checks(O, checks)
+    return out
+
+
+def fp64_oracle_checks(O, checks):
+    """|dLC2| of GPU results against the FP64 oracle (exact least squares, FP64 sums) on the
+    same inputs. kind "3d": volumes on the ultrasound grid, MR/CT warped; kind "sweep":
+    tracked frames, the MR warped onto each frame and the sums taken over all frames."""
     out = {}
-    for name, fn in (("cg_fp32", lambda a, b: O.lc2_cg(U, I, G, r, zlo=a, zhi=b)),
-                     ("exact_fp64", lambda a, b: O.lc2_exact(U, I, G, r, zlo=a, zhi=b))):
-        t1 = timed(fn, 1)
-        k = int(max(1, min(planes, budget_s / max(t1, 1e-6))))
-        tk = timed(fn, k) if k > 1 else t1
-        t_eval = t_ws + tk * planes / k
-        out[name] = (1.0 / t_eval, k, t_eval)
-        log(f"cpu {name}: {k} of {planes} planes in {tk:.2f}s -> {t_eval:.2f} s/eval")
-    threads = O.num_threads()
-    v, k, t_eval = out["cg_fp32"]
-    return {"value": v, "unit": "LC2 evals/s", "cores": threads, "kind": "port",
-            "sample": (f"reference algorithm (FP32 CGLS, <=20 it) restated in oracle/, "
-                       f"{threads} OpenMP threads: full warps + Sobel, CGLS on {k} of {planes} "
-                       f"output planes, scaled; {t_eval:.2f} s per evaluation"),
-            "exact_fp64_evals_per_s": out["exact_fp64"][0]}
+    for name, w, kind, mats, gpu, stored in checks:
+        t = time.perf_counter()
+        f, m = w.fixed, w.moving
+        mg = O.geom(m.dims, m.spacing, m.origin)
+        sources = {"source": m.data.astype(np.float32)}
+        if stored is not None:
+            sources["stored"] = np.asarray(stored, np.float32)
+        res, U = {}, None
+        if kind == "3d":
+            fg, _, vg, _ = _oracle_grids(O, w)
+            U = O.warp_3d(f.data.astype(np.float32), fg, vg, [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0])
+        for src_name, src in sources.items():
+            l64 = []
+            for M in mats:
+                if kind == "3d":
+                    I = O.warp_3d(src, mg, vg, np.asarray(M).ravel()[:12])
+                    l64.append(O.lc2_exact(U, I, O.sobel(I, "sep"), w.patch_radius).lc2)
+                else:
+                    X, Y = f.dims[0], f.dims[1]
+                    vg = O.geom((X, Y, 1), (f.spacing[0], f.spacing[1], 1.0), (0, 0, 0))
+                    sm = sw = 0.0
+                    for k in range(M.shape[0]):
+                        I = O.warp_3d(src, mg, vg, M[k].ravel())[0]
+                        r = O.lc2_exact(f.data[k].astype(np.float32), I, O.sobel(I, "sep"),
+                                        w.patch_radius)
+                        sm += r.sum_measure
+                        sw += r.sum_w
+                    l64.append(sm / sw if sw > 0 else 0.0)
+            res[src_name] = np.asarray(l64)
+        g = np.asarray(gpu, np.float64)
+        out[name] = {"poses": len(mats),
+                     "lc2_gpu": [float(x) for x in g],
+                     "lc2_fp64_oracle": [float(x) for x in res["source"]],
+                     "max_abs_dlc2_vs_fp64": float(np.abs(g - res["source"]).max())}
+        if "stored" in res:
+            out[name]["max_abs_dlc2_vs_fp64_on_stored_values"] = float(
+                np.abs(g - res["stored"]).max())
+        log(f"FP64 oracle check {name}: |dLC2| {out[name]['max_abs_dlc2_vs_fp64']:.2e} "
+            f"({time.perf_counter() - t:.1f}s)")
+    return out
+
+
+def batching_figures(eng, E, synth, w, pair, fv, mats, res, batched_f32, checks):
+    """The batching and precision forms against the plain single-pose FP32 form (one pose
+    per launch, launched eagerly), on the 3D affine workload and on a 256x256 2D affine
+    case: batched launches replayed from captured graphs, with MR stored f32 and f16. The
+    f16 results join `checks` for the FP64 oracle."""
+    m = w.moving
+    P = len(mats)
+    out = {}
+
+    def rate(fn, n, reps):
+        fn()
+        eng.sync()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return n * reps / (time.perf_counter() - t)
+
+    pair.use_graph(False)
+    k = iter(range(10 ** 9))
+    single = rate(lambda: pair.eval(mats[next(k) % P]), 1, 20)
+    pair.use_graph(True)
+    single_graph = rate(lambda: pair.eval_batch(mats[next(k) % P][None]), 1, 20)
+    mv16 = eng.upload(m.data, m.spacing, m.origin, E.F16)
+    p16 = eng.pair(fv, mv16, w.resampled_spacing, True, False, w.patch_radius, 3)
+    p16.use_graph(True)
+    r16 = p16.eval_batch(mats)
+    batched16 = rate(lambda: p16.eval_batch(mats), P, 10)
+    p16.close()
+    mv16.close()
+    checks.append(("3d_affine_mr_f16_storage", w, "3d", mats[:2], r16.lc2[:2],
+                   m.data.astype(np.float16)))
+    out["batching_3d"] = {
+        "single_pose_fp32_evals_per_s": single,
+        "single_pose_fp32_graph_evals_per_s": single_graph,
+        "batched_fp32_evals_per_s": batched_f32,
+        "batched_fp16_evals_per_s": batched16,
+        "batched_fp32_over_single_fp32": batched_f32 / single,
+        "batched_fp16_over_single_fp32": batched16 / single,
+        "fp16_vs_fp32_max_abs_dlc2": float(np.abs(r16.lc2 - res.lc2).max()),
+        "note": "one pose of the 192x192x160 grid already fills the GPU, so batching saves "
+                "only per-launch overhead and the partial last wave; f16 storage does not "
+                "speed up a kernel bound by vector ALU, and rounding the MR to f16 moves LC2 "
+                "(FP64 oracle check)"}
+    log(f"3D: single-pose fp32 {single:.0f} evals/s (graph {single_graph:.0f}), batched f32 "
+        f"{batched_f32:.0f}, batched f16 {batched16:.0f} evals/s")
+    # 2D 256x256 affine (ultrasound warped onto the static image): one pose is a few
+    # hundred work-groups, far from filling the GPU, so the batch carries the throughput
+    w0 = synth.config0(seed=1)
+    f0, m0 = w0.fixed, w0.moving
+    a = eng.upload(f0.data, f0.spacing, f0.origin)
+    b = eng.upload(m0.data, m0.spacing, m0.origin)
+    p2 = eng.pair(a, b, w0.resampled_spacing, w0.high_res_is_moving, w0.center_overlaid,
+                  w0.patch_radius, 2)
+    rng = np.random.default_rng(2)
+    m2 = np.stack([p2.pose_to_matrix(synth.perturb(w0.true_pose, rng, 2)) for _ in range(64)])
+    s2 = rate(lambda: p2.eval(m2[next(k) % 64]), 1, 200)
+    p2.use_graph(True)
+    s2g = rate(lambda: p2.eval_batch(m2[next(k) % 64][None]), 1, 200)
+    b2 = rate(lambda: p2.eval_batch(m2), 64, 50)
+    out["batching_2d_256"] = {"single_pose_fp32_evals_per_s": s2,
+                              "single_pose_fp32_graph_evals_per_s": s2g,
+                              "batched_64_fp32_evals_per_s": b2,
+                              "batched_over_single": b2 / s2,
+                              "config": "256x256 f32 fixed vs u8 moving affine, 7x7 patch, "
+                                        "64 poses per graph-replayed launch"}
+    log(f"2D 256^2: single-pose {s2:.0f} evals/s (graph {s2g:.0f}), batched 64 {b2:.0f} evals/s")
+    for h in (p2, a, b):
+        h.close()
+    return out
 
 
 def sweep_figures(eng, E, synth, steps=3, registration=True):
@@ -89,6 +257,7 @@ def sweep_figures(eng, E, synth, steps=3, registration=True):
     mv = eng.upload(m.data, m.spacing, m.origin, E.F16)
     p = eng.pair(fv, mv, w.resampled_spacing, True, False, w.patch_radius, 2)
     p.set_per_frame(True)
+    p.use_graph(True)
     rng = np.random.default_rng(7)
     poses = [w.true_pose] + [w.true_pose + np.r_[rng.uniform(-2, 2, 3), rng.uniform(-2, 2, 3)]
                              for _ in range(63)]
@@ -97,6 +266,8 @@ def sweep_figures(eng, E, synth, steps=3, registration=True):
     res = p.eval_batch(mats)
     eng.sync()
     p.profile(True)
+    p.eval_batch(mats)   # capture with the timing events
+    p.profile(True)      # counters from zero, the captured graph kept
     t = time.perf_counter()
     for _ in range(steps):
         res = p.eval_batch(mats)
@@ -105,11 +276,17 @@ def sweep_figures(eng, E, synth, steps=3, registration=True):
     p.profile(False)
     X, Y, F = p.virt_dims
     px = P * F * X * Y
+    t_k = kms / max(nl, 1) / 1e3
+    t_valu = px * (ALG_F32_SWEEP / FP32_PEAK + ALG_F64_PER_VOXEL / FP64_PEAK)
     out = {"sweep_evals_per_s": P / dt, "sweep_pixel_poses_per_s": px / dt,
-           "sweep_kernel_ms_per_launch": kms / max(nl, 1),
+           "sweep_kernel_ms_per_launch": t_k * 1e3,
+           "sweep_roofline": {"bound": "valu", "achieved": _f32_equiv(px, ALG_F32_SWEEP) / t_k / 1e12,
+                              "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s", "frac": t_valu / t_k},
            "sweep_config": f"{F} frames {X}x{Y} u8 (0.3 mm) vs MR 320x320x256 f16, 7x7 patch, "
-                           f"{P} poses per launch",
+                           f"{P} poses per graph-replayed launch",
            "sweep_lc2_at_true_pose": float(res.lc2[0])}
+    check = ("sweep_mr_f16_storage", w, "sweep", mats[:1], res.lc2[:1],
+             m.data.astype(np.float16))
     log(f"sweep: {out['sweep_evals_per_s']:.1f} whole-sweep evals/s "
         f"({out['sweep_pixel_poses_per_s'] / 1e9:.2f} G pixel-poses/s)")
     if registration:
@@ -117,7 +294,7 @@ def sweep_figures(eng, E, synth, steps=3, registration=True):
     p.close()
     fv.close()
     mv.close()
-    return out
+    return out, check
 
 
 def sweep_registration_figures(p, w, synth, n_runs=3, seed=43):
@@ -344,6 +521,113 @@ def serving_figures(eng, E, synth, pair3d, mats3d, w_ct):
     return out
 
 
+def registration_serving_figures(eng, E, synth, w3d, w_ct, rates=(5, 20), duration_s=8.0):
+    """Whole registrations as requests (lc2s_registrar over the serving loop): open-loop
+    Poisson arrivals of registrations drawn from the serving mix -- 40% 2D affine 256x256 /
+    512x512 (6 parameters, from the generating pose moved by up to 2 px / 2 deg / 0.02),
+    40% rigid tracked-sweep registrations of 32 / 64 frames against one MR 256x256x192 f16,
+    20% rigid 3D (US 192x192x160 vs MR 256x256x192 f32; US 256^3 vs CT 512x512x400 f16),
+    the last two from the generating pose moved by up to 2 mm / 2 deg. Each is admitted
+    (its pair built) when a session is free, runs BOBYQA (xtol_rel 1e-4) with every
+    candidate set one serving request, and frees its pair. Returns per-class p50 / p99
+    latency from arrival to result, evaluations, and registrations/s."""
+    from concurrent.futures import ThreadPoolExecutor
+    from lc2registration_amd import serve as S
+    keep, classes = [], []
+    rng = np.random.default_rng(71)
+    for seed, n in ((41, 256), (43, 512)):
+        w = synth.config0(seed=seed, n=n)
+        f, m = w.fixed, w.moving
+        fv, mv = eng.upload(f.data, f.spacing, f.origin), eng.upload(m.data, m.spacing, m.origin)
+        keep += [fv, mv]
+        half = np.r_[5, 5, 5, 0.08, 0.08, 0.05]
+
+        def make(w=w, fv=fv, mv=mv, half=half):
+            x0 = w.true_pose + np.r_[rng.uniform(-2, 2, 3), rng.uniform(-0.02, 0.02, 2), 0]
+            return dict(fixed=fv, moving=mv, resampled_spacing=w.resampled_spacing,
+                        high_res_is_moving=w.high_res_is_moving,
+                        center_overlaid=w.center_overlaid, patch_radius=w.patch_radius, dim=2,
+                        x0=x0, lower=x0 - half, upper=x0 + half), w.true_pose
+        classes.append((f"2d_affine_{n}", make, 0.2))
+    mr = None
+    for frames in (32, 64):
+        w = synth.config2(seed=22, mr_dims=(256, 256, 192), n_frames=frames)
+        if mr is None:
+            mr = eng.upload(w.moving.data, w.moving.spacing, w.moving.origin, E.F16)
+            keep.append(mr)
+        fv = eng.upload(w.fixed.data, w.fixed.spacing, w.fixed.origin)
+        keep.append(fv)
+
+        def make(w=w, fv=fv):
+            truth = np.asarray(w.true_pose, np.float64)[:6]
+            x0 = truth + rng.uniform(-2, 2, 6)
+            return dict(fixed=fv, moving=mr, resampled_spacing=w.resampled_spacing,
+                        high_res_is_moving=True, center_overlaid=False,
+                        patch_radius=w.patch_radius, dim=2, x0=x0, lower=x0 - 4, upper=x0 + 4,
+                        frames=w.extra["frames"], frames_centre=w.extra["centre"]), truth
+        classes.append((f"sweep_{frames}_frames", make, 0.2))
+    for name, w in (("3d_mr_us192x192x160", w3d), ("3d_ct_us256", w_ct)):
+        f, m = w.fixed, w.moving
+        fv, mv = eng.upload(f.data, f.spacing, f.origin), eng.upload(m.data, m.spacing, m.origin)
+        keep += [fv, mv]
+
+        def make(w=w, fv=fv, mv=mv):
+            truth = np.asarray(w.true_pose, np.float64)[:6]
+            x0 = truth + rng.uniform(-2, 2, 6)
+            return dict(fixed=fv, moving=mv, resampled_spacing=w.resampled_spacing,
+                        high_res_is_moving=True, center_overlaid=False,
+                        patch_radius=w.patch_radius, dim=3, x0=x0, lower=x0 - 4,
+                        upper=x0 + 4), truth
+        classes.append((name, make, 0.1))
+    out = {"config": "registrations drawn from the serving mix; BOBYQA, xtol_rel 1e-4; "
+                     "8 sessions, 4 launch slots, 64 poses per merged launch"}
+    weights = np.array([c[2] for c in classes])
+    for lam in rates:
+        srv = S.Server(eng, max_poses=64, slots=4)
+        reg = S.Registrar(srv, eng, max_sessions=8)
+        n = max(int(lam * duration_s), 1)
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, n))
+        kind = rng.choice(len(classes), n, p=weights / weights.sum())
+        lat = [[] for _ in classes]
+        nev = [[] for _ in classes]
+        err = [[] for _ in classes]
+
+        def collect(ticket, c, t_arr, truth):
+            x, _, ne, _, _ = reg.wait(ticket)
+            lat[c].append(time.perf_counter() - t_arr)
+            nev[c].append(ne)
+            err[c].append(float(np.abs(x[:3] - truth[:3]).max()))
+
+        with ThreadPoolExecutor(max_workers=64) as pool:
+            t0 = time.perf_counter()
+            for a, c in zip(arrivals, kind):
+                now = time.perf_counter() - t0
+                if a > now:
+                    time.sleep(a - now)
+                kw, truth = classes[c][1]()
+                pool.submit(collect, reg.submit(**kw), c, t0 + a, truth)
+        dt = time.perf_counter() - t0
+        st = reg.stats()
+        reg.close()
+        srv.close()
+        r = {"registrations_per_s": st["completed"] / dt, "completed": st["completed"],
+             "failed": st["failed"], "max_running": st["max_running"]}
+        for (name, _, _), l, ne, e in zip(classes, lat, nev, err):
+            ms = np.array(l) * 1e3
+            r[name] = {"registrations": len(l),
+                       "p50_ms": float(np.percentile(ms, 50)) if len(l) else None,
+                       "p99_ms": float(np.percentile(ms, 99)) if len(l) else None,
+                       "median_evaluations": float(np.median(ne)) if ne else None,
+                       "max_translation_error": max(e) if e else None}
+        out[f"{lam}_per_s"] = r
+        log(f"registration serving {lam}/s: " + ", ".join(
+            f"{nm} {r[nm]['p50_ms']:.0f}/{r[nm]['p99_ms']:.0f} ms" for nm, _, _ in classes
+            if r[nm]["registrations"]) + f"; up to {st['max_running']} running")
+    for h in keep:
+        h.close()
+    return {"serve_registrations": out}
+
+
 def ct_precision_figures(eng, E, synth, w, poses=32, steps=3):
     """CT precision workload: int16 CT 512x512x400 (0.8 mm, HU) against a 256^3 uint8
     ultrasound (0.6 mm), 7^3 patch, 32 poses per launch. The CT stored as half (exact for
@@ -354,23 +638,41 @@ def ct_precision_figures(eng, E, synth, w, poses=32, steps=3):
     out, res = {}, {}
     rng = np.random.default_rng(31)
     qs = [w.true_pose] + [synth.perturb(w.true_pose, rng, 3) for _ in range(poses - 1)]
-    for name, storage in (("f16", E.STORE_AUTO), ("f32", E.F32)):
+    mats = None
+    for name, storage, nbytes in (("f16", E.STORE_AUTO, 2), ("f32", E.F32, 4)):
         u0, _ = eng.memory()
         mv = eng.upload(m.data, m.spacing, m.origin, storage)
         u1, _ = eng.memory()
         p = eng.pair(fv, mv, w.resampled_spacing, True, False, w.patch_radius, 3)
+        p.use_graph(True)
         mats = np.stack([p.pose_to_matrix(q) for q in qs])
         res[name] = p.eval_batch(mats)
         eng.sync()
         p.profile(True)
+        p.eval_batch(mats)   # capture with the timing events
+        p.profile(True)      # counters from zero, the captured graph kept
         t = time.perf_counter()
         for _ in range(steps):
             p.eval_batch(mats)
         dt = (time.perf_counter() - t) / steps
         kms, nl = p.profile_read()
+        t_k = kms / max(nl, 1) / 1e3
+        V = int(np.prod(p.virt_dims))
+        # algorithmic bytes per pose: the u8 ultrasound once, and the CT region inside the
+        # ultrasound field of view at the CT's own resolution and storage width
+        fov = V * float(np.prod(np.asarray(w.resampled_spacing) / np.asarray(m.spacing)))
+        alg = len(mats) * (V * 1.0 + fov * nbytes)
+        t_valu = len(mats) * V * (ALG_F32_PER_VOXEL / FP32_PEAK + ALG_F64_PER_VOXEL / FP64_PEAK)
         out[f"ct_{name}_evals_per_s"] = len(mats) / dt
-        out[f"ct_{name}_kernel_ms_per_launch"] = kms / max(nl, 1)
+        out[f"ct_{name}_kernel_ms_per_launch"] = t_k * 1e3
         out[f"ct_{name}_volume_bytes"] = u1 - u0
+        out[f"ct_{name}_roofline"] = {
+            "bound": "valu" if t_valu >= alg / (HBM_PEAK_GBS * 1e9) else "hbm",
+            "hbm": {"achieved": alg / t_k / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": alg / t_k / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": alg},
+            "valu": {"achieved": _f32_equiv(len(mats) * V) / t_k / 1e12, "peak": FP32_PEAK / 1e12,
+                     "unit": "TFLOP/s", "frac": t_valu / t_k}}
+        p.profile(False)
         p.close()
         mv.close()
     fv.close()
@@ -378,10 +680,12 @@ def ct_precision_figures(eng, E, synth, w, poses=32, steps=3):
     out["ct_f16_vs_f32_max_abs_dlc2"] = float(np.abs(res["f16"].lc2 - res["f32"].lc2).max())
     out["ct_lc2_at_true_pose"] = float(res["f16"].lc2[0])
     out["ct_config"] = (f"CT int16 512x512x400 (0.8 mm, HU {int(m.data.min())}..{int(m.data.max())}) "
-                        f"vs US 256^3 u8 (0.6 mm), 7^3 patch, {poses} poses per launch")
+                        f"vs US 256^3 u8 (0.6 mm), 7^3 patch, {poses} poses per graph-replayed "
+                        f"launch")
     log(f"CT: f16 {out['ct_f16_evals_per_s']:.0f} evals/s, f32 {out['ct_f32_evals_per_s']:.0f} "
         f"evals/s, LC2 bit-identical: {out['ct_f16_lc2_bit_identical_to_f32']}")
-    return out
+    check = ("ct_f16_storage", w, "3d", mats[:2], res["f16"].lc2[:2], None)
+    return out, check
 
 
 def registration_3d_figures(w, n_runs=3, seed=41):
@@ -457,6 +761,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds per CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip single-pose / fp16 figures")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch each step eagerly instead of replaying its captured graph")
     ap.add_argument("--sweep-only", action="store_true",
                     help="only the tracked-sweep figure (2D kernel), timed over --steps launches "
                          "(for profiling that kernel)")
@@ -471,7 +777,7 @@ def main():
         from lc2registration_amd import engine as E
         from lc2registration_amd import synth
         print(json.dumps(sweep_figures(E.Engine(0), E, synth, steps=args.steps,
-                                       registration=False)), flush=True)
+                                       registration=False)[0]), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1",
@@ -501,11 +807,16 @@ def main():
         zc = autotune.tune(pair, mats)
         log(f"autotuned z-chunk {zc}")
 
+    # the product path: each step replays the HIP graph captured for this batch size
+    # (matrix upload, fused kernel, fixed-order reduction, result download)
+    pair.use_graph(not args.no_graph)
     for _ in range(args.warmup):
         pair.eval_batch(mats)
     eng.sync()
+    pair.profile(True)      # timing events inside the graph: capture once more with them
+    pair.eval_batch(mats)
+    pair.profile(True)      # counters from zero, the captured graph kept
     log(f"warmup done ({args.warmup} steps of {P} poses); timing {args.steps} steps")
-    pair.profile(True)
     lat = []
     eng.sync()
     t_start = time.perf_counter()
@@ -523,13 +834,13 @@ def main():
     log(f"{value:.1f} evals/s, kernel {avg_kernel_s * 1e3:.3f} ms per launch of {P} poses")
 
     alg_bytes = P * V * ALG_BYTES_PER_VOXEL
-    achieved = alg_bytes / avg_kernel_s / 1e9
+    achieved_gbs = alg_bytes / avg_kernel_s / 1e9
     t_valu = P * V * (ALG_F32_PER_VOXEL / FP32_PEAK + ALG_F64_PER_VOXEL / FP64_PEAK)
     traffic, valu_insts, tr = None, None, {}
     if os.path.exists(PROFILE_TRAFFIC):
         with open(PROFILE_TRAFFIC) as fh:
             tr = json.load(fh)
-        if tr.get("poses") == P:
+        if tr.get("poses") == P and tr.get("mr_storage", "f32") == args.mr_storage:
             traffic = tr.get("hbm_bytes_per_launch")
             valu_insts = tr.get("sq_insts_valu_per_launch")
         else:
@@ -540,19 +851,26 @@ def main():
         if not a or not b:
             return None
         return 1.0 - a / b if complement else a / b
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+    # The binding roof is the vector ALU: achieved = algorithmic FP32 + FP64 work in FP32
+    # equivalents per second, so frac = t_VALU / t_kernel.
+    roofline = {"bound": "valu", "achieved": _f32_equiv(P * V) / avg_kernel_s / 1e12,
+                "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s", "frac": t_valu / avg_kernel_s,
+                "traffic": traffic,
                 "kernel": "k_fused3d<R=3, moving high-res>",
                 "kernel_ms_per_launch": avg_kernel_s * 1e3, "launches": launches,
-                "algorithmic_bytes_per_launch": alg_bytes,
-                "valu_frac": t_valu / avg_kernel_s,
-                "valu_note": "algorithmic FP32 116 + FP64 45 flop per voxel-pose vs "
-                             "157.3 / 78.6 TFLOP/s vector peaks",
-                # the binding resource: VALU issue. SQ_INSTS_VALU of the profiled launch
-                # (profiles/), 4 cycles per wave64 instruction on a 16-lane SIMD, over
-                # 1024 SIMDs at 2.4 GHz
+                "algorithmic_flops_per_launch": {"f32": P * V * ALG_F32_PER_VOXEL,
+                                                 "f64": P * V * ALG_F64_PER_VOXEL},
+                "valu_note": "algorithmic FP32 116 + FP64 45 flop per voxel-pose; t_VALU = "
+                             "F32 / 157.3 T + F64 / 78.6 T",
+                "hbm": {"achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved_gbs / HBM_PEAK_GBS,
+                        "algorithmic_bytes_per_launch": alg_bytes, "traffic": traffic,
+                        "traffic_gbs": traffic / avg_kernel_s / 1e9 if traffic else None},
+                # VALU issue: SQ_INSTS_VALU of the profiled launch, 4 cycles per wave64
+                # instruction on a 16-lane SIMD, over 1024 SIMDs at 2.4 GHz
                 "valu_issue_frac": (valu_insts * 4 / (SIMDS * CLOCK_HZ) / avg_kernel_s
                                     if valu_insts else None),
+                "valu_insts_per_voxel_pose": (valu_insts * 64 / (P * V) if valu_insts else None),
                 # cache behaviour of the profiled launch (TCC / TCP / SQ counters)
                 "l2_hit_rate": (tr["tcc_hit_per_launch"] / (tr["tcc_hit_per_launch"] +
                                                             tr["tcc_miss_per_launch"])
@@ -562,28 +880,10 @@ def main():
                 "lds_conflict_cycle_frac": ratio("sq_lds_bank_conflict_per_launch",
                                                  "sq_active_inst_lds_per_launch")}
 
-    extra = {}
+    stored = m.data.astype(np.float16) if args.mr_storage == "f16" else None
+    extra, checks = {}, [(f"3d_affine_mr_{args.mr_storage}", w, "3d", mats[:2], res.lc2[:2], stored)]
     if not args.no_extra:
-        # plain single-pose FP32 form and the batched FP16-storage form
-        n1 = 10
-        pair.eval(mats[0])
-        t = time.perf_counter()
-        for i in range(n1):
-            pair.eval(mats[i % P])
-        extra["single_pose_fp32_evals_per_s"] = n1 / (time.perf_counter() - t)
-        mv16 = eng.upload(m.data, m.spacing, m.origin, E.F16)
-        p16 = eng.pair(fv, mv16, w.resampled_spacing, True, False, w.patch_radius, 3)
-        p16.eval_batch(mats)
-        t = time.perf_counter()
-        r16 = None
-        for _ in range(5):
-            r16 = p16.eval_batch(mats)
-        extra["batched_fp16_evals_per_s"] = 5 * P / (time.perf_counter() - t)
-        extra["fp16_vs_fp32_max_abs_dlc2"] = float(np.abs(r16.lc2 - res.lc2).max())
-        p16.close()
-        mv16.close()
-        log(f"single-pose fp32 {extra['single_pose_fp32_evals_per_s']:.1f} evals/s, "
-            f"batched fp16 {extra['batched_fp16_evals_per_s']:.1f} evals/s")
+        extra.update(batching_figures(eng, E, synth, w, pair, fv, mats, res, value, checks))
         # the mutual-information metric (patchSize 0, 256 x 256 bins) on the same pair
         pair.eval_mi(mats)
         t = time.perf_counter()
@@ -596,16 +896,20 @@ def main():
         t0 = time.time()
         w_ct = synth.config3()
         log(f"CT workload synthesised in {time.time() - t0:.1f}s")
-        extra.update(ct_precision_figures(eng, E, synth, w_ct))
+        ct, ck = ct_precision_figures(eng, E, synth, w_ct)
+        extra.update(ct)
+        checks.append(ck)
         extra.update(serving_figures(eng, E, synth, pair, mats, w_ct))
-        del w_ct
-        extra.update(sweep_figures(eng, E, synth))
+        extra.update(registration_serving_figures(eng, E, synth, w, w_ct))
+        sw, ck = sweep_figures(eng, E, synth)
+        extra.update(sw)
+        checks.append(ck)
         extra.update(registration_latency_figures(synth))
         extra.update(registration_3d_figures(w))
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(w, mats[0], args.cpu_budget)
+        cpu = cpu_baseline(w, mats[0], args.cpu_budget, checks)
 
     lat_ms = np.array(lat) * 1e3
     out = {
@@ -624,10 +928,10 @@ def main():
         "config": {"workload": f"3D affine LC2: MR 256x256x192 {args.mr_storage} (1 mm, moving) vs "
                                "US 192x192x160 u8 (0.5 mm, fixed grid), patch 7^3",
                    "poses_per_step": P, "virtual_grid": list(pair.virt_dims),
-                   "zchunk": zchunk_batch},
+                   "zchunk": zchunk_batch, "graph_replay": not args.no_graph},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "hbm_gbs_algorithmic": achieved,
+        "hbm_gbs_algorithmic": achieved_gbs,
         "step_latency_ms": {"p50": float(np.percentile(lat_ms, 50)),
                             "p99": float(np.percentile(lat_ms, 99))},
         "lc2_at_true_pose": float(res.lc2[0]),
