@@ -134,6 +134,19 @@ int lc2e_eval(lc2e_pair* p, const float* m12, double* lc2);
 int lc2e_eval_batch(lc2e_pair* p, int n, const float* mats, double* lc2, double* sum_measure,
                     double* sum_w);
 
+/* lc2e_eval_batch without any allocation: no counterpart in the reference. For callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). The launch's matrices, partial sums and results live in a fixed
+ * device array of the library (16 MiB per device, held until the process ends) instead of
+ * the pair's batch buffers; the call uses no hipMalloc/hipFree and no allocating library
+ * routine. Same kernels, so the results are bit-identical to lc2e_eval_batch. A call needs
+ * n*(24 + 48*mats_per_pose) + n*16*(partial sums per pose) bytes (each part rounded up to
+ * 256); a larger call fails with LC2E_ERR_INVALID. Every call on a device shares that
+ * device's array: concurrent calls would race on it, so the library serialises them per
+ * device (a call waits for the one in progress). */
+int lc2e_eval_batch_noalloc(lc2e_pair* p, int n, const float* mats, double* lc2,
+                            double* sum_measure, double* sum_w);
+
 /* Stream-ordered form for serving loops: d_mats and d_out are DEVICE pointers,
  * d_out receives 3 doubles per pose (sum_measure, sum_w, lc2). `stream` is a hipStream_t
  * (NULL = the engine's stream). Nothing is synchronised. The pair's partial-sum buffer is

@@ -54,6 +54,10 @@ struct lc2e_engine {
     int device = 0;
     int cus = 0;  // compute units of the device
     hipStream_t stream = nullptr;
+    // the device's scratch array for lc2e_eval_batch_noalloc (null if its lookup failed,
+    // then scratch_error says why)
+    void* scratch = nullptr;
+    std::string scratch_error;
     size_t budget = 0;  // 0 = unlimited
     size_t used = 0;
     std::mutex mu;      // guards `used`: a serving loop allocates from its own thread
@@ -467,6 +471,13 @@ int lc2e_create(int device, size_t hbm_budget_bytes, lc2e_engine** out) {
         delete e;
         return fail(LC2E_ERR_HIP, "stream creation failed: %s", hipGetErrorString(err));
     }
+    // the allocation-free entry point's device scratch: its address on this device, looked
+    // up once per device (with the device current) and kept
+    const hipError_t se = lc2e::scratch_address(device, &e->scratch);
+    if (se != hipSuccess) {
+        e->scratch = nullptr;
+        e->scratch_error = hipGetErrorString(se);
+    }
     *out = e;
     return LC2E_OK;
 }
@@ -730,6 +741,64 @@ int lc2e_eval_batch(lc2e_pair* p, int n, const float* mats, double* lc2, double*
 
 int lc2e_eval(lc2e_pair* p, const float* m12, double* lc2) {
     return lc2e_eval_batch(p, 1, m12, lc2, nullptr, nullptr);
+}
+
+int lc2e_eval_batch_noalloc(lc2e_pair* p, int n, const float* mats, double* lc2,
+                            double* sum_measure, double* sum_w) {
+    if (!p || !mats || n < 0) return fail(LC2E_ERR_INVALID, "invalid argument");
+    if (n == 0) return LC2E_OK;
+    lc2e_engine* e = p->e;
+    if (!e->scratch)
+        return fail(LC2E_ERR_HIP, "device scratch unavailable: %s", e->scratch_error.c_str());
+    // One device array serves every call on the device: calls are serialised here, so a
+    // call that overlaps another one on the same device waits for it.
+    static std::mutex device_locks[64];
+    std::lock_guard<std::mutex> lk(device_locks[e->device & 63]);
+    HIP_TRY(hipSetDevice(e->device));
+    hipStream_t st = e->stream;
+    const int mpp = p->mats_per_pose();
+    const Tiling t = tiling_for(p, n);
+    // scratch layout: results [n][3] f64, matrices [n][mpp][12] f32, partials [n][bpp][2] f64
+    auto up256 = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t out_b = up256((size_t)n * 3 * sizeof(double));
+    const size_t mat_b = up256((size_t)n * mpp * 12 * sizeof(float));
+    const size_t part_b = (size_t)n * (t.tiles > 0 ? t.bpp : 0) * 2 * sizeof(double);
+    if (out_b + mat_b + part_b > lc2e::kScratchBytes)
+        return fail(LC2E_ERR_INVALID,
+                    "%d poses need %zu bytes of device scratch; the allocation-free entry point "
+                    "has %zu (use lc2e_eval_batch or fewer poses per call)",
+                    n, out_b + mat_b + part_b, lc2e::kScratchBytes);
+    char* base = static_cast<char*>(e->scratch);
+    double* d_out = reinterpret_cast<double*>(base);
+    float* d_mats = reinterpret_cast<float*>(base + out_b);
+    double* d_part = reinterpret_cast<double*>(base + out_b + mat_b);
+    HIP_TRY(hipMemcpyAsync(d_mats, mats, (size_t)n * mpp * 12 * sizeof(float),
+                           hipMemcpyHostToDevice, st));
+    if (t.tiles > 0) {
+        // every partial slot is written by the fused kernel before k_reduce reads it, so the
+        // previous call's contents need no clearing
+        const lc2e::FusedArgs a = make_args(p, d_mats, d_part, t);
+        HIP_TRY(launch_eval(p, a, n, st));
+    }
+    HIP_TRY(lc2e::launch_reduce(d_part, t.tiles > 0 ? t.bpp : 0, n, d_out, st));
+    // results back through a fixed host buffer, a chunk of poses at a time
+    int status = LC2E_OK;
+    constexpr int kChunk = 128;
+    double h[kChunk * 3];
+    for (int b = 0; b < n; b += kChunk) {
+        const int m = n - b < kChunk ? n - b : kChunk;
+        HIP_TRY(hipMemcpyAsync(h, d_out + (size_t)b * 3, (size_t)m * 3 * sizeof(double),
+                               hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (int i = 0; i < m; ++i) {
+            const double sm = h[3 * i], sw = h[3 * i + 1], l = h[3 * i + 2];
+            if (lc2) lc2[b + i] = l;
+            if (sum_measure) sum_measure[b + i] = sm;
+            if (sum_w) sum_w[b + i] = sw;
+            if (!(sw > 0.0) || !std::isfinite(sm / sw)) status = LC2E_OK_DEGENERATE;
+        }
+    }
+    return status;
 }
 
 int lc2e_eval_batch_async(lc2e_pair* p, int n, const float* d_mats, double* d_out, void* stream) {

@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <mutex>
 #include <type_traits>
 
@@ -1923,6 +1924,33 @@ hipError_t launch_mi(const FusedArgs& a, int grid_ndim, int src_ndim, int movi, 
     hipLaunchKernelGGL(k_mi_entropy, dim3(n_poses), dim3(256), 0, st, hist, ext_mov, ext_stat,
                        bins_u, bins_i, (double)n, out);
     return hipGetLastError();
+}
+
+// The scratch array of lc2e_eval_batch_noalloc. In host code its name is only a stand-in
+// for the symbol: kernels get the device address that scratch_address() looks up.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+hipError_t scratch_address(int device, void** out) {
+    static std::mutex mu;
+    static std::map<int, void*> addr;  // device -> address of its copy of the array
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = addr.find(device);
+    if (it != addr.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    int cur = 0;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess) return e;
+    if (!ptr) return hipErrorInvalidSymbol;
+    addr[device] = ptr;
+    *out = ptr;
+    return hipSuccess;
 }
 
 hipError_t launch_convert(const void* d_in, int in_dtype, void* d_out, int out_f16, long long n,

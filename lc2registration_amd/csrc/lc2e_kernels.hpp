@@ -97,6 +97,12 @@ int sep_blocks(const VGeom& v, int r, int grid_ndim);
 hipError_t launch_sep(const FusedArgs& a, int grid_ndim, int src_ndim, int movi, int n_poses,
                       hipStream_t st);
 constexpr int kSepChannels(bool movi) { return movi ? 7 : 10; }
+// Device scratch of the allocation-free batch entry point (lc2e_eval_batch_noalloc): one
+// file-scope device array of kScratchBytes per device, holding a launch's matrices,
+// partial sums and results. scratch_address() returns its address on `device` (looked up
+// once per device with that device current, then cached).
+constexpr size_t kScratchBytes = size_t(16) << 20;
+hipError_t scratch_address(int device, void** out);
 // Fixed image -> virtual grid at identity (frames of a 2D stack one by one).
 hipError_t launch_warp_static(const void* fixed, int fixed_f16, const SGeom& fg, const VGeom& v,
                               int grid_ndim, int src_ndim, float* out, hipStream_t st);

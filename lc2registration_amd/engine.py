@@ -70,6 +70,7 @@ def lib():
         L.lc2e_pose_to_matrix.argtypes = [vp, dp, C.c_int, fp]
         L.lc2e_eval.argtypes = [vp, fp, dp]
         L.lc2e_eval_batch.argtypes = [vp, C.c_int, fp, dp, dp, dp]
+        L.lc2e_eval_batch_noalloc.argtypes = [vp, C.c_int, fp, dp, dp, dp]
         L.lc2e_eval_batch_async.argtypes = [vp, C.c_int, vp, vp, vp]
         L.lc2e_sync.argtypes = [vp]
         L.lc2e_debug_fetch.argtypes = [vp, C.c_int, fp, vp]
@@ -216,6 +217,18 @@ class Pair:
         l, sm, sw = np.zeros(n), np.zeros(n), np.zeros(n)
         rc = _check(lib().lc2e_eval_batch(self.h, n, _fptr(m.ravel()), _dptr(l), _dptr(sm),
                                           _dptr(sw)))
+        return BatchResult(l, sm, sw, rc)
+
+    def eval_batch_noalloc(self, mats) -> BatchResult:
+        """eval_batch through the allocation-free entry point (device scratch array)."""
+        m = np.ascontiguousarray(mats, np.float32)
+        per = 12 * (self.info.n_frames if self.info.per_frame_matrices else 1)
+        n = m.size // per
+        if n * per != m.size:
+            raise ValueError("matrix buffer size is not a multiple of the per-pose size")
+        l, sm, sw = np.zeros(n), np.zeros(n), np.zeros(n)
+        rc = _check(lib().lc2e_eval_batch_noalloc(self.h, n, _fptr(m.ravel()), _dptr(l),
+                                                  _dptr(sm), _dptr(sw)))
         return BatchResult(l, sm, sw, rc)
 
     def debug_fetch(self, stage: int, m12) -> np.ndarray:
