@@ -43,6 +43,18 @@ def _f32_equiv(units, f32=ALG_F32_PER_VOXEL, f64=ALG_F64_PER_VOXEL):
     return units * (f32 + f64 * FP32_PEAK / FP64_PEAK)
 
 
+def _kernel_time(p, mats, launches):
+    """Fused-kernel time of `launches` launches of this batch: HIP events around the kernel
+    on the engine's stream (timed launches are not graph-replayed, so the events bracket
+    the kernel alone). Returns (total ms, launches)."""
+    p.profile(True)
+    for _ in range(launches):
+        p.eval_batch(mats)
+    out = p.profile_read()
+    p.profile(False)
+    return out
+
+
 def cpu_model():
     """The host CPU's model name (as lscpu reports it) and the cores this process may use."""
     model = None
@@ -176,28 +188,30 @@ def fp64_oracle_checks(O, checks):
 def batching_figures(eng, E, synth, w, pair, fv, mats, res, batched_f32, checks):
     """The batching and precision forms against the plain single-pose FP32 form (one pose
     per launch, launched eagerly), on the 3D affine workload and on a 256x256 2D affine
-    case: batched launches replayed from captured graphs, with MR stored f32 and f16. The
-    f16 results join `checks` for the FP64 oracle."""
+    case: batched launches with the MR stored f32 and f16, each launched eagerly and
+    replayed from a captured HIP graph. The f16 results join `checks` for the FP64 oracle."""
     m = w.moving
     P = len(mats)
     out = {}
 
     def rate(fn, n, reps):
         fn()
+        fn()   # with graphs on, the second call captures
         eng.sync()
         t = time.perf_counter()
         for _ in range(reps):
             fn()
         return n * reps / (time.perf_counter() - t)
 
-    pair.use_graph(False)
     k = iter(range(10 ** 9))
+    pair.use_graph(False)
     single = rate(lambda: pair.eval(mats[next(k) % P]), 1, 20)
     pair.use_graph(True)
     single_graph = rate(lambda: pair.eval_batch(mats[next(k) % P][None]), 1, 20)
+    batched_graph = rate(lambda: pair.eval_batch(mats), P, 10)
+    pair.use_graph(False)
     mv16 = eng.upload(m.data, m.spacing, m.origin, E.F16)
     p16 = eng.pair(fv, mv16, w.resampled_spacing, True, False, w.patch_radius, 3)
-    p16.use_graph(True)
     r16 = p16.eval_batch(mats)
     batched16 = rate(lambda: p16.eval_batch(mats), P, 10)
     p16.close()
@@ -208,6 +222,7 @@ def batching_figures(eng, E, synth, w, pair, fv, mats, res, batched_f32, checks)
         "single_pose_fp32_evals_per_s": single,
         "single_pose_fp32_graph_evals_per_s": single_graph,
         "batched_fp32_evals_per_s": batched_f32,
+        "batched_fp32_graph_evals_per_s": batched_graph,
         "batched_fp16_evals_per_s": batched16,
         "batched_fp32_over_single_fp32": batched_f32 / single,
         "batched_fp16_over_single_fp32": batched16 / single,
@@ -217,7 +232,7 @@ def batching_figures(eng, E, synth, w, pair, fv, mats, res, batched_f32, checks)
                 "speed up a kernel bound by vector ALU, and rounding the MR to f16 moves LC2 "
                 "(FP64 oracle check)"}
     log(f"3D: single-pose fp32 {single:.0f} evals/s (graph {single_graph:.0f}), batched f32 "
-        f"{batched_f32:.0f}, batched f16 {batched16:.0f} evals/s")
+        f"{batched_f32:.0f} (graph {batched_graph:.0f}), batched f16 {batched16:.0f} evals/s")
     # 2D 256x256 affine (ultrasound warped onto the static image): one pose is a few
     # hundred work-groups, far from filling the GPU, so the batch carries the throughput
     w0 = synth.config0(seed=1)
@@ -229,16 +244,19 @@ def batching_figures(eng, E, synth, w, pair, fv, mats, res, batched_f32, checks)
     rng = np.random.default_rng(2)
     m2 = np.stack([p2.pose_to_matrix(synth.perturb(w0.true_pose, rng, 2)) for _ in range(64)])
     s2 = rate(lambda: p2.eval(m2[next(k) % 64]), 1, 200)
+    b2 = rate(lambda: p2.eval_batch(m2), 64, 50)
     p2.use_graph(True)
     s2g = rate(lambda: p2.eval_batch(m2[next(k) % 64][None]), 1, 200)
-    b2 = rate(lambda: p2.eval_batch(m2), 64, 50)
+    b2g = rate(lambda: p2.eval_batch(m2), 64, 50)
     out["batching_2d_256"] = {"single_pose_fp32_evals_per_s": s2,
                               "single_pose_fp32_graph_evals_per_s": s2g,
                               "batched_64_fp32_evals_per_s": b2,
+                              "batched_64_fp32_graph_evals_per_s": b2g,
                               "batched_over_single": b2 / s2,
                               "config": "256x256 f32 fixed vs u8 moving affine, 7x7 patch, "
-                                        "64 poses per graph-replayed launch"}
-    log(f"2D 256^2: single-pose {s2:.0f} evals/s (graph {s2g:.0f}), batched 64 {b2:.0f} evals/s")
+                                        "64 poses per launch"}
+    log(f"2D 256^2: single-pose {s2:.0f} evals/s (graph {s2g:.0f}), batched 64 {b2:.0f} "
+        f"(graph {b2g:.0f}) evals/s")
     for h in (p2, a, b):
         h.close()
     return out
@@ -257,7 +275,6 @@ def sweep_figures(eng, E, synth, steps=3, registration=True):
     mv = eng.upload(m.data, m.spacing, m.origin, E.F16)
     p = eng.pair(fv, mv, w.resampled_spacing, True, False, w.patch_radius, 2)
     p.set_per_frame(True)
-    p.use_graph(True)
     rng = np.random.default_rng(7)
     poses = [w.true_pose] + [w.true_pose + np.r_[rng.uniform(-2, 2, 3), rng.uniform(-2, 2, 3)]
                              for _ in range(63)]
@@ -265,15 +282,11 @@ def sweep_figures(eng, E, synth, steps=3, registration=True):
     P = len(mats)
     res = p.eval_batch(mats)
     eng.sync()
-    p.profile(True)
-    p.eval_batch(mats)   # capture with the timing events
-    p.profile(True)      # counters from zero, the captured graph kept
     t = time.perf_counter()
     for _ in range(steps):
         res = p.eval_batch(mats)
     dt = (time.perf_counter() - t) / steps
-    kms, nl = p.profile_read()
-    p.profile(False)
+    kms, nl = _kernel_time(p, mats, steps)
     X, Y, F = p.virt_dims
     px = P * F * X * Y
     t_k = kms / max(nl, 1) / 1e3
@@ -283,7 +296,7 @@ def sweep_figures(eng, E, synth, steps=3, registration=True):
            "sweep_roofline": {"bound": "valu", "achieved": _f32_equiv(px, ALG_F32_SWEEP) / t_k / 1e12,
                               "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s", "frac": t_valu / t_k},
            "sweep_config": f"{F} frames {X}x{Y} u8 (0.3 mm) vs MR 320x320x256 f16, 7x7 patch, "
-                           f"{P} poses per graph-replayed launch",
+                           f"{P} poses per launch",
            "sweep_lc2_at_true_pose": float(res.lc2[0])}
     check = ("sweep_mr_f16_storage", w, "sweep", mats[:1], res.lc2[:1],
              m.data.astype(np.float16))
@@ -644,18 +657,14 @@ def ct_precision_figures(eng, E, synth, w, poses=32, steps=3):
         mv = eng.upload(m.data, m.spacing, m.origin, storage)
         u1, _ = eng.memory()
         p = eng.pair(fv, mv, w.resampled_spacing, True, False, w.patch_radius, 3)
-        p.use_graph(True)
         mats = np.stack([p.pose_to_matrix(q) for q in qs])
         res[name] = p.eval_batch(mats)
         eng.sync()
-        p.profile(True)
-        p.eval_batch(mats)   # capture with the timing events
-        p.profile(True)      # counters from zero, the captured graph kept
         t = time.perf_counter()
         for _ in range(steps):
             p.eval_batch(mats)
         dt = (time.perf_counter() - t) / steps
-        kms, nl = p.profile_read()
+        kms, nl = _kernel_time(p, mats, steps)
         t_k = kms / max(nl, 1) / 1e3
         V = int(np.prod(p.virt_dims))
         # algorithmic bytes per pose: the u8 ultrasound once, and the CT region inside the
@@ -672,7 +681,6 @@ def ct_precision_figures(eng, E, synth, w, poses=32, steps=3):
                     "frac": alg / t_k / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": alg},
             "valu": {"achieved": _f32_equiv(len(mats) * V) / t_k / 1e12, "peak": FP32_PEAK / 1e12,
                      "unit": "TFLOP/s", "frac": t_valu / t_k}}
-        p.profile(False)
         p.close()
         mv.close()
     fv.close()
@@ -680,8 +688,7 @@ def ct_precision_figures(eng, E, synth, w, poses=32, steps=3):
     out["ct_f16_vs_f32_max_abs_dlc2"] = float(np.abs(res["f16"].lc2 - res["f32"].lc2).max())
     out["ct_lc2_at_true_pose"] = float(res["f16"].lc2[0])
     out["ct_config"] = (f"CT int16 512x512x400 (0.8 mm, HU {int(m.data.min())}..{int(m.data.max())}) "
-                        f"vs US 256^3 u8 (0.6 mm), 7^3 patch, {poses} poses per graph-replayed "
-                        f"launch")
+                        f"vs US 256^3 u8 (0.6 mm), 7^3 patch, {poses} poses per launch")
     log(f"CT: f16 {out['ct_f16_evals_per_s']:.0f} evals/s, f32 {out['ct_f32_evals_per_s']:.0f} "
         f"evals/s, LC2 bit-identical: {out['ct_f16_lc2_bit_identical_to_f32']}")
     check = ("ct_f16_storage", w, "3d", mats[:2], res["f16"].lc2[:2], None)
@@ -761,11 +768,15 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds per CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip single-pose / fp16 figures")
-    ap.add_argument("--no-graph", action="store_true",
-                    help="launch each step eagerly instead of replaying its captured graph")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay each step from a captured HIP graph instead of launching it "
+                         "eagerly (on this stack a replay costs 5-13 us more per batch)")
     ap.add_argument("--sweep-only", action="store_true",
                     help="only the tracked-sweep figure (2D kernel), timed over --steps launches "
                          "(for profiling that kernel)")
+    ap.add_argument("--ct-only", action="store_true",
+                    help="only the CT precision figures (f16 and f32 storage), timed over --steps "
+                         "launches (for profiling that workload)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES three times (the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -778,6 +789,12 @@ def main():
         from lc2registration_amd import synth
         print(json.dumps(sweep_figures(E.Engine(0), E, synth, steps=args.steps,
                                        registration=False)[0]), flush=True)
+        return
+    if args.ct_only:
+        from lc2registration_amd import engine as E
+        from lc2registration_amd import synth
+        print(json.dumps(ct_precision_figures(E.Engine(0), E, synth, synth.config3(),
+                                              steps=args.steps)[0]), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1",
@@ -807,15 +824,13 @@ def main():
         zc = autotune.tune(pair, mats)
         log(f"autotuned z-chunk {zc}")
 
-    # the product path: each step replays the HIP graph captured for this batch size
-    # (matrix upload, fused kernel, fixed-order reduction, result download)
-    pair.use_graph(not args.no_graph)
-    for _ in range(args.warmup):
+    # the product path: each step uploads the matrices, launches the fused kernel and the
+    # fixed-order reduction, and downloads the results (eagerly, or replayed from a HIP
+    # graph with --graph)
+    pair.use_graph(args.graph)
+    for _ in range(max(args.warmup, 2)):   # the second call captures the graph
         pair.eval_batch(mats)
     eng.sync()
-    pair.profile(True)      # timing events inside the graph: capture once more with them
-    pair.eval_batch(mats)
-    pair.profile(True)      # counters from zero, the captured graph kept
     log(f"warmup done ({args.warmup} steps of {P} poses); timing {args.steps} steps")
     lat = []
     eng.sync()
@@ -826,8 +841,7 @@ def main():
         lat.append(time.perf_counter() - ts)
     eng.sync()
     elapsed = time.perf_counter() - t_start
-    kernel_ms, launches = pair.profile_read()
-    pair.profile(False)
+    kernel_ms, launches = _kernel_time(pair, mats, args.steps)
     zchunk_batch = pair.get_tuning()
     avg_kernel_s = kernel_ms / max(launches, 1) / 1e3
     value = args.steps * P / elapsed
@@ -928,7 +942,7 @@ def main():
         "config": {"workload": f"3D affine LC2: MR 256x256x192 {args.mr_storage} (1 mm, moving) vs "
                                "US 192x192x160 u8 (0.5 mm, fixed grid), patch 7^3",
                    "poses_per_step": P, "virtual_grid": list(pair.virt_dims),
-                   "zchunk": zchunk_batch, "graph_replay": not args.no_graph},
+                   "zchunk": zchunk_batch, "graph_replay": args.graph},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "hbm_gbs_algorithmic": achieved_gbs,

@@ -192,16 +192,19 @@ int lc2e_pair_autotune(lc2e_pair* p, int n, const float* mats, int iters, int* b
                        double* best_ms);
 
 /* Kernel timing with HIP events recorded around every fused-kernel launch on the engine's
- * stream (in a HIP graph as event-record nodes, timed on every replay). enable = 1 resets
- * the counters. read returns the summed fused-kernel time in ms and the number of launches
- * since the reset. */
+ * stream. While timing is on, batches are launched eagerly even where graph replay is on
+ * (the events bracket the kernel itself); replay resumes when it is turned off. enable = 1
+ * resets the counters. read returns the summed fused-kernel time in ms and the number of
+ * launches since the reset. */
 int lc2e_pair_profile(lc2e_pair* p, int enable);
 int lc2e_pair_profile_read(lc2e_pair* p, double* kernel_ms, long long* launches);
 
 /* Record the batched evaluation of n poses (matrix upload, kernels, result download) in a
- * HIP graph on its first lc2e_eval_batch with n poses and replay it for later calls with the
- * same n (1 = on, 0 = off). Graphs are dropped when the pair's buffers or launch shape
- * change. The registration facade turns this on for its pairs. */
+ * HIP graph on its second lc2e_eval_batch with n poses and replay it for later calls with
+ * the same n (1 = on, 0 = off, the default). Graphs are dropped when the pair's buffers or
+ * launch shape change. Results are identical either way. On MI355X with ROCm 7.2 a
+ * replayed batch took 5-13 us longer than an eager one at 1 to 64 poses, so nothing in
+ * this library turns it on by itself (the facade does with LC2E_GRAPHS=1). */
 int lc2e_pair_use_graph(lc2e_pair* p, int enable);
 
 #ifdef __cplusplus

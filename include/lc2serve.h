@@ -63,10 +63,11 @@ int lc2s_wait(lc2s_server* s, long long ticket, double timeout_ms, double* lc2,
 /* Counters since creation. */
 int lc2s_stats(lc2s_server* s, lc2s_counters* out);
 
-/* Graph replay of launches (on by default): each slot runs a (pair, poses) launch eagerly
+/* Graph replay of launches (off by default): each slot runs a (pair, poses) launch eagerly
  * the first time, captures it (matrix upload, kernels, result download) as a HIP graph the
  * second time and replays the graph after that, until the pair's buffers or launch shape
- * change. Results are identical either way. enable = 0 launches eagerly. */
+ * change. Results are identical either way. Off by default because on MI355X with ROCm 7.2
+ * a replayed launch took 5-13 us longer than an eager one (measured at 1 to 64 poses). */
 int lc2s_use_graphs(lc2s_server* s, int enable);
 
 /* ---- Whole registrations as requests ----------------------------------------------

@@ -369,7 +369,7 @@ int enqueue_batch(lc2e_pair* p, int n, const Tiling& t, hipStream_t st) {
                            hipMemcpyHostToDevice, st));
     if (t.tiles > 0) {
         lc2e::FusedArgs a = make_args(p, p->d_mats, p->d_partials, t);
-        // in a captured graph the events become event-record nodes, timed on every replay
+        // timed launches are eager (run_batch), the events bracket the fused kernel
         const bool timed = p->profile;
         if (timed) HIP_TRY(hipEventRecord(p->ev0, st));
         HIP_TRY(launch_eval(p, a, n, st));
@@ -387,7 +387,7 @@ int enqueue_batch(lc2e_pair* p, int n, const Tiling& t, hipStream_t st) {
 // Runs a batch of n poses staged in h_mats; the buffers must already fit tiling t.
 int run_batch(lc2e_pair* p, int n, const Tiling& t) {
     hipStream_t st = p->e->stream;
-    if (p->use_graph) {
+    if (p->use_graph && !p->profile) {
         auto it = p->graphs.find(n);
         if (it == p->graphs.end()) {
             GraphEntry g;
@@ -1016,10 +1016,7 @@ int lc2e_pair_get_tuning(lc2e_pair* p, int* zchunk) {
 
 int lc2e_pair_profile(lc2e_pair* p, int enable) {
     if (!p) return fail(LC2E_ERR_INVALID, "null pair");
-    if ((enable != 0) != p->profile) {
-        (void)hipStreamSynchronize(p->e->stream);
-        drop_graphs(p);  // graphs are captured with or without the timing events
-    }
+    if ((enable != 0) != p->profile) (void)hipStreamSynchronize(p->e->stream);
     if (enable && !p->ev0) {
         HIP_TRY(hipEventCreate(&p->ev0));
         HIP_TRY(hipEventCreate(&p->ev1));

@@ -883,18 +883,27 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
                 SC += cc - ringc[slot];
                 ringc[slot] = cc;
             }
+            // pair sums: slot (q-1) turns from plane q-1 into Q(q-1) = P(q-1) + P(q), slot q
+            // takes P(q); a window of W = 2R+1 planes ending at q is then the R pair sums
+            // Q(q-2R), Q(q-2R+2), ..., Q(q-2) plus P(q): R+1 adds per channel, not 2R
+            constexpr int prev = (slot + W - 1) % W;
+            double Pn[NC];
 #pragma unroll
-            for (int c = 0; c < NC; ++c) ring[slot][c] = s_prod[c * TX * TY + tid];
+            for (int c = 0; c < NC; ++c) {
+                Pn[c] = s_prod[c * TX * TY + tid];
+                ring[prev][c] = ring[prev][c] + Pn[c];
+                ring[slot][c] = Pn[c];
+            }
             if (q >= 2 * R) {
                 const double mu = mu_pre, cyy = cyy_pre;
                 if (q + 1 < nout + 2 * R) issue_stats(zp - R + 1);
                 double S[NC];
 #pragma unroll
                 for (int c = 0; c < NC; ++c) {
-                    double t = ring[(slot + 1) % W][c];  // oldest plane
+                    double t = ring[(slot + 1) % W][c];  // Q(q-2R)
 #pragma unroll
-                    for (int j = 2; j <= W; ++j) t += ring[(slot + j) % W][c];
-                    S[c] = t;
+                    for (int j = 1; j < R; ++j) t += ring[(slot + 1 + 2 * j) % W][c];
+                    S[c] = t + Pn[c];
                 }
                 if (col_valid) {
                     double w, meas;
@@ -1005,9 +1014,8 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
 
     using Pend = typename std::conditional<SRC3D, Pending3<T>, Pending2<T>>::type;
     Pend pend[SPT];
-    // waves whose positions of sample / products position j all lie past the region skip
-    // it (wave-uniform, as in the 3D kernel)
-    const int wbase = __builtin_amdgcn_readfirstlane(tid & ~63);
+    // (No wave-uniform skip of empty slots here, unlike the 3D kernel: in this kernel it
+    // cost 6% on the tracked sweep, 29.09 -> 30.93 ms per launch, measured.)
     auto issue_frame = [&](int frame) {
         float m[12];
         const float* mp = mats + (a.mats_per_pose > 1 ? frame : 0) * 12;
@@ -1016,7 +1024,6 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
 #pragma unroll
         for (int j = 0; j < SPT; ++j) {
             if (j * kThreads >= NI) break;  // uniform
-            if (j * kThreads + wbase >= NI) continue;  // wave-uniform
             const bool inside = (g_in >> j) & 1u;
             if constexpr (SRC3D) issue3(src, a.s, m, gLx[j], gLy[j], lz, inside, pend[j]);
             else issue2(src, a.s, m, gLx[j], gLy[j], inside, pend[j]);
@@ -1057,7 +1064,6 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
 #pragma unroll
         for (int j = 0; j < SPT; ++j) {
             if (j * kThreads >= NI) break;  // uniform
-            if (j * kThreads + wbase >= NI) continue;  // wave-uniform
             const int i = tid + j * kThreads;
             float v;
             if constexpr (SRC3D) v = finish3(pend[j]);
@@ -1081,7 +1087,6 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
             if (j * kThreads >= NP) break;  // uniform
-            if (j * kThreads + wbase >= NP) continue;  // wave-uniform
             const int i = tid + j * kThreads;
             if (i < NP) {
                 float I, G, U;

@@ -127,7 +127,7 @@ struct lc2s_server {
     // worker thread only
     std::set<lc2e_pair*> in_flight;  // pairs with a launch on the device
     long long extra_launches = 0;    // per-request launches of split merged launches
-    std::atomic<bool> use_graphs{true};        // replay captured launches (lc2s_use_graphs)
+    std::atomic<bool> use_graphs{false};       // replay captured launches (lc2s_use_graphs)
     std::atomic<long long> graph_launches{0};
 
     std::mutex mu;

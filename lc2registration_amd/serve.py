@@ -94,7 +94,7 @@ class Server:
         return _E.BatchResult(l, sm, sw, rc)
 
     def use_graphs(self, on: bool):
-        """Replay captured launches (default on) or launch eagerly."""
+        """Replay captured launches or launch eagerly (the default)."""
         _E._check(lib().lc2s_use_graphs(self.h, int(on)))
 
     def stats(self):

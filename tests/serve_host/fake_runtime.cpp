@@ -8,6 +8,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdlib>
+#include <cstdint>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -122,6 +123,65 @@ long long fake_used() {
 
 // --- engine entry points the serving loop uses -----------------------------------------
 const char* lc2e_last_error(void) { return g_err.c_str(); }
+
+// --- and those the registrar (lc2e_sessions.cpp) uses ----------------------------------
+// A pair "created" from two fake volumes charges g_pair_bytes to the budget; its id is the
+// creation count. Its matrices hold the pose parameters in their first floats.
+namespace {
+long long g_pair_bytes = 0;
+int g_pairs_created = 0, g_pairs_live = 0, g_pairs_max_live = 0;
+}
+void fake_config_pairs(long long pair_bytes) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_pair_bytes = pair_bytes;
+    g_pairs_created = g_pairs_live = g_pairs_max_live = 0;
+}
+int fake_pairs_max_live() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return g_pairs_max_live;
+}
+int fake_pairs_live() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return g_pairs_live;
+}
+int lc2e_pair_create(lc2e_engine*, lc2e_volume* fixed, lc2e_volume*, const float*, int, int, int,
+                     int, lc2e_pair** out) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_budget && g_used + (size_t)g_pair_bytes > g_budget) {
+        g_err = "fake HBM budget exceeded by a pair";
+        return LC2E_ERR_BUDGET;
+    }
+    g_used += (size_t)g_pair_bytes;
+    auto* p = new lc2e_pair();
+    p->id = 1000 + ++g_pairs_created;
+    // frames: the fixed volume's "address" encodes the frame count (fake volumes are tags)
+    p->mpp = (int)(reinterpret_cast<uintptr_t>(fixed) & 0xff);
+    if (p->mpp < 1) p->mpp = 1;
+    p->share = 0.05;
+    ++g_pairs_live;
+    if (g_pairs_live > g_pairs_max_live) g_pairs_max_live = g_pairs_live;
+    *out = p;
+    return LC2E_OK;
+}
+void lc2e_pair_destroy(lc2e_pair* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_used -= (size_t)g_pair_bytes;
+    --g_pairs_live;
+    delete p;
+}
+int lc2e_pair_set_per_frame(lc2e_pair* p, int on) {
+    if (!on) p->mpp = 1;
+    return LC2E_OK;
+}
+int lc2e_pose_to_matrix(lc2e_pair*, const double* params, int n, float out12[12]) {
+    if (n < 1 || n > 12) {
+        g_err = "fake pose_to_matrix: bad parameter count";
+        return LC2E_ERR_INVALID;
+    }
+    for (int k = 0; k < 12; ++k) out12[k] = k < n ? (float)params[k] : 0.0f;
+    return LC2E_OK;
+}
 
 int lc2e_pair_get_info(lc2e_pair* p, lc2e_pair_info* info) {
     std::memset(info, 0, sizeof(*info));

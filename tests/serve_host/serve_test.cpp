@@ -283,6 +283,7 @@ void scenario_graph_replay() {
     lc2e_pair* a = fake_pair(1, 1, 0.1, 0);
     lc2s_server* s = nullptr;
     CHECK(lc2s_create(e, 16, 1, 0.0, &s) == LC2E_OK);
+    CHECK(lc2s_use_graphs(s, 1) == LC2E_OK);  // off by default
     auto run = [&](int k) {
         std::vector<float> m = poses(3, 1, 10.0f * k);
         long long t = 0;

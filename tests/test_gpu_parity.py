@@ -272,6 +272,14 @@ def test_graph_replay_matches_eager(engine, w3):
         g = p.eval_batch(mats).lc2
         assert np.array_equal(g, eager), k
         assert p.eval(mats[k]) == eager[k]  # other work between replays
+    # kernel timing with replay on: timed launches run eagerly, replay resumes afterwards
+    p.profile(True)
+    for _ in range(3):
+        assert np.array_equal(p.eval_batch(mats).lc2, eager)
+    ms, n = p.profile_read()
+    p.profile(False)
+    assert n == 3 and ms > 0.0
+    assert np.array_equal(p.eval_batch(mats).lc2, eager)
     p.use_graph(False)
     p.close()
 
