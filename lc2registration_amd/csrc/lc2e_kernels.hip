@@ -18,10 +18,12 @@
 // 3D structure (k_fused3d): a 256-thread work-group (two per CU) owns a 16x16 column
 // of output voxels over `zchunk` planes and marches along z. Per input plane:
 // warp the plane (+ Sobel halo) into LDS, separable Sobel of each 3x3 neighbourhood
-// with a 3-plane register ring along z, FP64 products into LDS,
-// sliding x then y box sums in LDS, and a (2R+1)-plane ring of plane sums in
-// registers gives the z box sum; complete patch sums are combined in FP64 and
-// accumulated per thread. No intermediate volume touches HBM.
+// with a 3-plane register ring along z, FP64 products into LDS, x then y window sums in
+// LDS (van Herk / Gil-Werman: no partial sum holds a value from outside its window), and
+// a (2R+1)-slot register ring of plane sums and pair sums of planes gives the z window
+// sum; complete patch sums are combined in FP64 and accumulated per thread. No
+// intermediate volume touches HBM. Patch radii beyond the fused kernels run the
+// separable path (k_sep_pass / k_sep_combine) through HBM.
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
