@@ -125,9 +125,10 @@ struct lc2e_pair {
     int part_bpp = 0;  // partial sums per pose the partials buffer holds
     float* d_mats = nullptr;
     double* d_partials = nullptr;
-    double* d_out = nullptr;
-    float* h_mats = nullptr;   // pinned
-    double* h_out = nullptr;   // pinned
+    float* h_mats = nullptr;   // pinned, mapped and coherent
+    double* h_out = nullptr;   // pinned, mapped and coherent
+    float* h_mats_dev = nullptr;   // their device addresses
+    double* h_out_dev = nullptr;
     size_t cap_bytes = 0;
     bool use_graph = false;
     std::map<int, GraphEntry> graphs;
@@ -296,14 +297,14 @@ void free_batch(lc2e_pair* p) {
     lc2e_engine* e = p->e;
     if (p->d_mats) e->release(p->d_mats, (size_t)p->cap * p->mats_per_pose() * 12 * sizeof(float));
     if (p->d_partials) e->release(p->d_partials, (size_t)p->cap * p->part_bpp * 2 * sizeof(double));
-    if (p->d_out) e->release(p->d_out, (size_t)p->cap * 3 * sizeof(double));
     if (p->h_mats) (void)hipHostFree(p->h_mats);
     if (p->h_out) (void)hipHostFree(p->h_out);
     p->d_mats = nullptr;
     p->d_partials = nullptr;
-    p->d_out = nullptr;
     p->h_mats = nullptr;
     p->h_out = nullptr;
+    p->h_mats_dev = nullptr;
+    p->h_out_dev = nullptr;
     p->cap = 0;
     p->part_bpp = 0;
 }
@@ -337,22 +338,26 @@ int ensure_capacity(lc2e_pair* p, int n, int bpp) {
         p->d_mats = nullptr;
         return rc;
     }
-    if ((rc = e->alloc((void**)&p->d_out, (size_t)cap * 3 * sizeof(double))) < 0) {
-        e->release(p->d_mats, (size_t)cap * mpp * 12 * sizeof(float));
-        e->release(p->d_partials, (size_t)cap * bpp * 2 * sizeof(double));
-        p->d_mats = nullptr;
-        p->d_partials = nullptr;
-        return rc;
-    }
     p->cap = cap;  // free_batch returns the device bytes by these sizes
     p->part_bpp = bpp;
     void* hm = nullptr;
     void* ho = nullptr;
-    hipError_t he = hipHostMalloc(&hm, (size_t)cap * mpp * 12 * sizeof(float), hipHostMallocDefault);
+    // mapped and coherent: small launches read their matrices from here and every launch's
+    // reduction writes its results here, with no copy command (enqueue_batch)
+    const unsigned fl = hipHostMallocMapped | hipHostMallocCoherent;
+    hipError_t he = hipHostMalloc(&hm, (size_t)cap * mpp * 12 * sizeof(float), fl);
     if (he == hipSuccess) {
         p->h_mats = (float*)hm;
-        he = hipHostMalloc(&ho, (size_t)cap * 3 * sizeof(double), hipHostMallocDefault);
+        he = hipHostMalloc(&ho, (size_t)cap * 3 * sizeof(double), fl);
         if (he == hipSuccess) p->h_out = (double*)ho;
+    }
+    void* dm = nullptr;
+    void* dout = nullptr;
+    if (he == hipSuccess) he = hipHostGetDevicePointer(&dm, hm, 0);
+    if (he == hipSuccess) he = hipHostGetDevicePointer(&dout, ho, 0);
+    if (he == hipSuccess) {
+        p->h_mats_dev = (float*)dm;
+        p->h_out_dev = (double*)dout;
     }
     if (he != hipSuccess) {
         free_batch(p);  // no half-made buffers: the next call allocates again
@@ -395,25 +400,32 @@ lc2e::FusedArgs make_args(lc2e_pair* p, const float* d_mats, double* d_partials,
     return a;
 }
 
-// Enqueue: H->D of the staged matrices, fused kernel, reduction, D->H of the results.
+// Enqueue: the staged matrices to the device, the fused kernel, the reduction into the
+// pinned result buffer. Each launch command costs several microseconds, so small launches
+// (up to kDirectPoses poses of one matrix each, on a single volume or image: each
+// work-group reads its pose's 12 floats once) read the pinned matrices in place instead of
+// copying them, and the reduction always writes the results in place: one or two commands
+// per batch instead of four.
+constexpr int kDirectPoses = 8;
 int enqueue_batch(lc2e_pair* p, int n, const Tiling& t, hipStream_t st) {
     const int mpp = p->mats_per_pose();
-    HIP_TRY(hipMemcpyAsync(p->d_mats, p->h_mats, (size_t)n * mpp * 12 * sizeof(float),
-                           hipMemcpyHostToDevice, st));
+    const bool direct = !p->sep && mpp == 1 && n <= kDirectPoses &&
+                        (p->info.grid_ndim == 3 || p->v.Z == 1);
+    if (!direct)
+        HIP_TRY(hipMemcpyAsync(p->d_mats, p->h_mats, (size_t)n * mpp * 12 * sizeof(float),
+                               hipMemcpyHostToDevice, st));
     if (t.tiles > 0) {
-        lc2e::FusedArgs a = make_args(p, p->d_mats, p->d_partials, t);
+        lc2e::FusedArgs a = make_args(p, direct ? p->h_mats_dev : p->d_mats, p->d_partials, t);
         // timed launches are eager (run_batch), the events bracket the fused kernel
         const bool timed = p->profile;
         if (timed) HIP_TRY(hipEventRecord(p->ev0, st));
         HIP_TRY(launch_eval(p, a, n, st));
         if (timed) HIP_TRY(hipEventRecord(p->ev1, st));
-        HIP_TRY(lc2e::launch_reduce(p->d_partials, t.bpp, n, p->d_out, st));
+        HIP_TRY(lc2e::launch_reduce(p->d_partials, t.bpp, n, p->h_out_dev, st));
     } else {
         // no patch centre fits in the grid: every sum is zero
-        HIP_TRY(lc2e::launch_reduce(p->d_partials, 0, n, p->d_out, st));
+        HIP_TRY(lc2e::launch_reduce(p->d_partials, 0, n, p->h_out_dev, st));
     }
-    HIP_TRY(hipMemcpyAsync(p->h_out, p->d_out, (size_t)n * 3 * sizeof(double),
-                           hipMemcpyDeviceToHost, st));
     return LC2E_OK;
 }
 

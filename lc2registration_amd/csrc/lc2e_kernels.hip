@@ -600,11 +600,14 @@ __device__ __forceinline__ void window_run(const V* in, int stride, V* out, int 
 // PRIO_X / PRIO_Y: run the x / y pass at raised wave priority. Both fused kernels set them:
 // with several work-groups per CU out of phase, the LDS passes gain from going first
 // (3D +1.3%, 2D +3.2%, measured).
+struct NoTail {
+    __device__ void operator()() const {}
+};
 template <int NC, bool CNT, int TX, int TY, int NTHR, int W, int XS, bool PRIO_X = false,
-          bool PRIO_Y = false>
+          bool PRIO_Y = false, class Tail = NoTail>
 __device__ __forceinline__ void box_xy(const double* s_prod, const int* s_cnt, double* s_xp,
                                        int* s_xcnt, double* s_out, int* s_ocnt, int ps, int py,
-                                       int tid) {
+                                       int tid, Tail tail = Tail{}) {
     static_assert(TX % 16 == 0 && TY % 16 == 0, "tile edges are multiples of 16");
     static_assert(XS > TX, "x-pass rows hold TX outputs");
     constexpr int NT = NC + (CNT ? 1 : 0);
@@ -634,6 +637,7 @@ __device__ __forceinline__ void box_xy(const double* s_prod, const int* s_cnt, d
             window_run<W, int, 8>(s_xcnt + seg * 8 * XS + col, XS, s_ocnt + seg * 8 * TX + col, TX);
     }
     if constexpr (PRIO_Y) __builtin_amdgcn_s_setprio(0);
+    tail();  // work whose LDS writes the closing barrier publishes too
     __syncthreads();
 }
 
@@ -678,13 +682,23 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
     constexpr int LAG = MOVI ? 2 : 0;
     constexpr int NC = MOVI ? 7 : 9;
     constexpr bool CNT = !MOVI;
+    // EARLY: the next plane's gathers are finished into s_in before the y pass's closing
+    // barrier, which then also publishes s_in: three barriers per z-step instead of four
+    // (7^3 patch: 4.06 -> 3.99 ms per 32-pose launch, measured). The plane sums then need
+    // their own LDS (s_sum) instead of reusing s_prod, which the next step's products
+    // overwrite while the ring still reads; that fits two work-groups per CU with 7
+    // channels, not with 9. Not at R = 1, where the longer-lived gather registers cost a
+    // wave per SIMD (2.64 -> 3.44 ms, measured).
+    constexpr bool EARLY = MOVI && R >= 2;
 
     __shared__ float s_in[NI];
     __shared__ double s_prod[NC * PY * PS];
     __shared__ int s_cnt[CNT ? PY * PS : 1];
     __shared__ double s_xp[NC * PY * XS];
     __shared__ int s_xcnt[CNT ? PY * XS : 1];
+    __shared__ double s_sum[EARLY ? NC * TX * TY : 1];
     __shared__ double s_red[2 * NTHR / 64];
+    double* const s_out = EARLY ? s_sum : s_prod;
 
     int tile, pose;
     if (!map_block(a.tiles, a.npose, tile, pose)) return;  // whole work-group
@@ -736,6 +750,19 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
 #pragma unroll
         for (int j = 0; j < SPT; ++j)
             if (wave_has(j, NI)) issue3c(src, a.s, m, cxy[j], lz, zin && ((in_xy >> j) & 1u), pend[j]);
+    };
+    // finish the gathers of the plane in flight into s_in. The finishing wave runs at
+    // raised priority so that every wave reaches the next barrier sooner (+0.3%, measured).
+    auto finish_plane = [&]() {
+        __builtin_amdgcn_s_setprio(2);
+#pragma unroll
+        for (int j = 0; j < SPT; ++j) {
+            if (!wave_has(j, NI)) continue;
+            const int i = tid + j * NTHR;
+            const float v = finish3(pend[j]);
+            if (i < NI) s_in[i] = v;
+        }
+        __builtin_amdgcn_s_setprio(0);
     };
     // static image values (U if MOVI, else I and |grad I|) of the products positions
     float pre0[PPT], pre1[PPT];
@@ -789,6 +816,10 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
     issue_plane(zw0);
     issue_static(z0 - R);      // products plane of the first products step
     issue_stats(z0);           // first output plane
+    if (EARLY) {
+        finish_plane();
+        __syncthreads();
+    }
 
     for (int s0 = 0; s0 < nsteps; s0 += W) {
         // W consecutive steps with compile-time ring slots (k), so the z ring stays in VGPRs.
@@ -798,18 +829,8 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
             if (s >= nsteps) return;  // block-uniform
             const int q = s - LAG;
             const int zp = z0 - R + q;
-            // 1. finish the plane issued last step, start the next one. The finishing
-            // wave runs at raised priority so that every wave reaches the first barrier
-            // sooner (+0.3%, measured).
-            __builtin_amdgcn_s_setprio(2);
-#pragma unroll
-            for (int j = 0; j < SPT; ++j) {
-                if (!wave_has(j, NI)) continue;
-                const int i = tid + j * NTHR;
-                const float v = finish3(pend[j]);
-                if (i < NI) s_in[i] = v;
-            }
-            __builtin_amdgcn_s_setprio(0);
+            // 1. (not EARLY) finish the plane issued last step
+            if (!EARLY) finish_plane();
             float cur0[PPT], cur1[PPT];
 #pragma unroll
             for (int j = 0; j < PPT; ++j) {
@@ -817,7 +838,7 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
                 cur1[j] = pre1[j];
             }
             if (q >= 0 && q + 1 < nout + 2 * R) issue_static(zp + 1);
-            __syncthreads();
+            if (!EARLY) __syncthreads();
             // 2. in-plane Sobel of the 3x3 neighbourhood (rows: smooth s = (l + 2c) + r and
             // difference d = r - l along x; then A, Bx, By down the rows), a 3-plane ring
             // in registers along z, |grad| and the products of plane zp
@@ -870,12 +891,21 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
                 }
             }
             __syncthreads();  // s_in reads done; products visible
-            // the next plane's gathers fly during the box sums and the combine
-            if (s + 1 < nsteps) issue_plane(zw0 + s + 1);
-            if (q < 0) return;  // block-uniform: Sobel ring warm-up only
-            // 3. in-plane box sums
-            box_xy<NC, CNT, TX, TY, NTHR, W, XS, true, true>(s_prod, s_cnt, s_xp, s_xcnt, s_prod, s_cnt, PS,
-                                                 PY, tid);
+            // the next plane's gathers fly during the box sums (and, not EARLY, the combine)
+            const bool more = s + 1 < nsteps;
+            if (more) issue_plane(zw0 + s + 1);
+            if (q < 0) {  // block-uniform: Sobel ring warm-up only
+                if (EARLY) {
+                    if (more) finish_plane();
+                    __syncthreads();
+                }
+                return;
+            }
+            // 3. in-plane box sums (EARLY: then the next plane into s_in)
+            box_xy<NC, CNT, TX, TY, NTHR, W, XS, true, true>(
+                s_prod, s_cnt, s_xp, s_xcnt, s_out, s_cnt, PS, PY, tid, [&]() {
+                    if (EARLY && more) finish_plane();
+                });
             // 4. z ring of plane sums and combine. The patch sums add the W plane sums in
             // z order (no running sum), so a voxel's value does not depend on where its
             // work-group's z-chunk starts; integer counts can run.
@@ -892,7 +922,7 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
             double Pn[NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
-                Pn[c] = s_prod[c * TX * TY + tid];
+                Pn[c] = s_out[c * TX * TY + tid];
                 ring[prev][c] = ring[prev][c] + Pn[c];
                 ring[slot][c] = Pn[c];
             }
