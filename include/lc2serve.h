@@ -11,7 +11,8 @@
  * OpenIGTLink pose server (interpolator.cu:359-388), streams results and does not batch.
  *
  * Admission is by bytes. Volumes and pairs are created through lc2e.h against the engine's
- * HBM budget (LC2E_ERR_BUDGET when they do not fit), and so are the slots' staging buffers.
+ * HBM budget (LC2E_ERR_BUDGET when they do not fit), and so are the slots' matrix buffers
+ * (results land in mapped pinned host memory).
  * If a merged launch does not fit the budget, its requests are evaluated one by one.
  * Pairs are served in the order of their oldest queued request, and a pair's requests are
  * launched in the order they arrived; a pair has at most one launch in flight. Each request's values are those of lc2e_eval_batch on its own poses:
@@ -64,7 +65,7 @@ int lc2s_wait(lc2s_server* s, long long ticket, double timeout_ms, double* lc2,
 int lc2s_stats(lc2s_server* s, lc2s_counters* out);
 
 /* Graph replay of launches (off by default): each slot runs a (pair, poses) launch eagerly
- * the first time, captures it (matrix upload, kernels, result download) as a HIP graph the
+ * the first time, captures it (matrix upload and kernels) as a HIP graph the
  * second time and replays the graph after that, until the pair's buffers or launch shape
  * change. Results are identical either way. Off by default because on MI355X with ROCm 7.2
  * a replayed launch took 5-13 us longer than an eager one (measured at 1 to 64 poses). */

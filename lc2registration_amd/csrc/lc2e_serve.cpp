@@ -58,7 +58,7 @@ int pair_mats_per_pose(lc2e_pair* p) {
     return info.per_frame_matrices ? info.n_frames : 1;
 }
 
-// A launch captured as a HIP graph on a slot: upload, kernels, download for one pair and
+// A launch captured as a HIP graph on a slot: matrix upload and kernels for one pair and
 // batch size, valid while the pair's launch state keeps its generation.
 struct SlotGraph {
     unsigned long long gen = 0;
@@ -81,9 +81,9 @@ struct Slot {
     size_t mats_cap = 0;  // floats
     int out_cap = 0;      // poses
     float* d_mats = nullptr;
-    double* d_out = nullptr;
-    float* h_mats = nullptr;  // pinned
-    double* h_out = nullptr;  // pinned
+    float* h_mats = nullptr;      // pinned
+    double* h_out = nullptr;      // pinned, mapped and coherent: the reduction writes here
+    double* h_out_dev = nullptr;  // its device address
     bool busy = false;
     lc2e_pair* pair = nullptr;
     std::vector<RequestPtr> batch;
@@ -154,13 +154,12 @@ struct lc2s_server {
         for (auto& kv : s.graphs) destroy_graph(kv.second);  // they captured these buffers
         s.graphs.clear();
         if (s.d_mats) lc2e::engine_release(engine, s.d_mats, s.mats_cap * sizeof(float));
-        if (s.d_out) lc2e::engine_release(engine, s.d_out, (size_t)s.out_cap * 3 * sizeof(double));
         if (s.h_mats) (void)hipHostFree(s.h_mats);
         if (s.h_out) (void)hipHostFree(s.h_out);
         s.d_mats = nullptr;
-        s.d_out = nullptr;
         s.h_mats = nullptr;
         s.h_out = nullptr;
+        s.h_out_dev = nullptr;
         s.mats_cap = 0;
         s.out_cap = 0;
     }
@@ -178,17 +177,19 @@ struct lc2s_server {
             s.d_mats = nullptr;
         } else {
             s.mats_cap = fc;
-            rc = lc2e::engine_alloc(engine, (void**)&s.d_out, (size_t)nc * 3 * sizeof(double));
-            if (rc < 0) s.d_out = nullptr;
-            else s.out_cap = nc;
+            s.out_cap = nc;
         }
         if (rc >= 0) {
             void* hm = nullptr;
             void* ho = nullptr;
+            void* hod = nullptr;
             if (hipHostMalloc(&hm, fc * sizeof(float), hipHostMallocDefault) == hipSuccess) s.h_mats = (float*)hm;
-            if (hipHostMalloc(&ho, (size_t)nc * 3 * sizeof(double), hipHostMallocDefault) == hipSuccess)
+            if (hipHostMalloc(&ho, (size_t)nc * 3 * sizeof(double),
+                              hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess) {
                 s.h_out = (double*)ho;
-            if (!s.h_mats || !s.h_out) {
+                if (hipHostGetDevicePointer(&hod, ho, 0) == hipSuccess) s.h_out_dev = (double*)hod;
+            }
+            if (!s.h_mats || !s.h_out || !s.h_out_dev) {
                 lc2e::set_last_error("serving: pinned staging allocation failed");
                 rc = LC2E_ERR_HIP;
             }
@@ -259,21 +260,16 @@ struct lc2s_server {
 
     // Outside the lock: upload, launch and download on the slot's stream, then record its
     // event. The slot's staging and the pair's partial sums are not in use by the device.
-    // The launch itself: upload, lc2e_eval_batch_async, download on the slot's stream.
+    // The launch itself: upload and lc2e_eval_batch_async on the slot's stream; the
+    // reduction writes the results straight into the slot's mapped host buffer (one copy
+    // command less per launch).
     int enqueue_ops(Slot& s, int n, size_t floats) {
         if (hipMemcpyAsync(s.d_mats, s.h_mats, floats * sizeof(float), hipMemcpyHostToDevice,
                            s.stream) != hipSuccess) {
             lc2e::set_last_error("serving: matrix upload failed");
             return LC2E_ERR_HIP;
         }
-        const int rc = lc2e_eval_batch_async(s.pair, n, s.d_mats, s.d_out, s.stream);
-        if (rc < 0) return rc;
-        if (hipMemcpyAsync(s.h_out, s.d_out, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost,
-                           s.stream) != hipSuccess) {
-            lc2e::set_last_error("serving: result download failed");
-            return LC2E_ERR_HIP;
-        }
-        return LC2E_OK;
+        return lc2e_eval_batch_async(s.pair, n, s.d_mats, s.h_out_dev, s.stream);
     }
 
     // A (pair, n) launch runs eagerly the first time (which also sizes the pair's buffers

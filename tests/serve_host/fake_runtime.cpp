@@ -344,6 +344,10 @@ hipError_t hipHostMalloc(void** p, size_t bytes, unsigned int) {
     *p = std::malloc(bytes ? bytes : 1);
     return *p ? hipSuccess : hipErrorOutOfMemory;
 }
+hipError_t hipHostGetDevicePointer(void** dev, void* host, unsigned int) {
+    *dev = host;  // one address space here
+    return hipSuccess;
+}
 hipError_t hipHostFree(void* p) {
     std::free(p);
     return hipSuccess;
