@@ -641,20 +641,56 @@ __device__ __forceinline__ void box_xy(const double* s_prod, const int* s_cnt, d
     __syncthreads();
 }
 
-// box_xy for a patch radius known only at run time (the 2D kernel), 1 <= r <= RMAX, over
-// the (TX + 2r) x (TY + 2r) products region of row stride lds_stride_2d(TX + 2r): the strides
-// become compile-time constants of each radius.
-template <int NC, bool CNT, int TX, int TY, int NTHR, int RMAX = kMaxR2, int RR = 1>
-__device__ __forceinline__ void box_xy_r(int r, const double* s_prod, const int* s_cnt,
-                                         double* s_xp, int* s_xcnt, double* s_out, int* s_ocnt,
-                                         int tid) {
+// Row of output oy of the in-place box sums (box_xy_inplace) in the products planes.
+__host__ __device__ constexpr int box_row(int oy, int w) { return oy < 8 ? oy : oy + w - 1; }
+
+// Box sums in place (the 2D kernel), for 16x16 outputs: the x pass writes each row's 16
+// outputs over the start of that row of s_prod, and the y pass writes each column run's 8
+// outputs over rows that no other run reads: run 0 over rows 0..7, run 1 over rows
+// W+7 .. W+14 (the last 8 of its own input rows). A task reads each of its inputs before it
+// overwrites it, so output (ox, oy) of channel c ends up at
+// s_prod[c * ps * py + box_row(oy, W) * ps + ox] with no second buffer: the LDS of the x
+// pass's output and of the sums is saved, which lets more work-groups share a CU.
+template <int NC, bool CNT, int NTHR, int W, bool PRIO_X = false, bool PRIO_Y = false>
+__device__ __forceinline__ void box_xy_inplace(double* s_prod, int* s_cnt, int ps, int py,
+                                               int tid) {
+    constexpr int NT = NC + (CNT ? 1 : 0);
+    const int np = ps * py;
+    if constexpr (PRIO_X) __builtin_amdgcn_s_setprio(2);
+    for (int t = tid; t < NT * py; t += NTHR) {
+        const int row = t % py, c = t / py;
+        if (c < NC) {
+            double* r = s_prod + c * np + row * ps;
+            window_run<W>(r, 1, r, 1);
+        } else {
+            int* r = s_cnt + row * ps;
+            window_run<W>(r, 1, r, 1);
+        }
+    }
+    if constexpr (PRIO_X) __builtin_amdgcn_s_setprio(0);
+    __syncthreads();
+    if constexpr (PRIO_Y) __builtin_amdgcn_s_setprio(2);
+    for (int t = tid; t < NT * 16 * 2; t += NTHR) {
+        const int col = t % 16, rest = t / 16;
+        const int seg = rest % 2, c = rest / 2;
+        const int in_ofs = seg * 8 * ps + col, out_ofs = box_row(seg * 8, W) * ps + col;
+        if (c < NC)
+            window_run<W, double, 8>(s_prod + c * np + in_ofs, ps, s_prod + c * np + out_ofs, ps);
+        else
+            window_run<W, int, 8>(s_cnt + in_ofs, ps, s_cnt + out_ofs, ps);
+    }
+    if constexpr (PRIO_Y) __builtin_amdgcn_s_setprio(0);
+    __syncthreads();
+}
+
+// box_xy_inplace for a patch radius known only at run time, 1 <= r <= RMAX.
+template <int NC, bool CNT, int NTHR, int RMAX = kMaxR2, int RR = 1>
+__device__ __forceinline__ void box_xy_inplace_r(int r, double* s_prod, int* s_cnt, int tid) {
     if (r == RR)
-        box_xy<NC, CNT, TX, TY, NTHR, 2 * RR + 1, lds_stride_2d(TX + 1), true, true>(
-            s_prod, s_cnt, s_xp, s_xcnt, s_out, s_ocnt, lds_stride_2d(TX + 2 * RR), TY + 2 * RR,
-            tid);
+        box_xy_inplace<NC, CNT, NTHR, 2 * RR + 1, true, true>(s_prod, s_cnt,
+                                                             lds_stride_2d(16 + 2 * RR), 16 + 2 * RR, tid);
     else if constexpr (RR < RMAX)
-        box_xy_r<NC, CNT, TX, TY, NTHR, RMAX, RR + 1>(r, s_prod, s_cnt, s_xp, s_xcnt, s_out, s_ocnt,
-                                                      tid);
+        box_xy_inplace_r<NC, CNT, NTHR, RMAX, RR + 1>(r, s_prod, s_cnt, tid);
 }
 
 // ---------------------------------------------------------------------------
@@ -980,6 +1016,7 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
 template <bool MOVI, bool SRC3D, typename T, int RMAX>
 __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
     extern __shared__ double s_dyn[];
+    static_assert(kTile2X == 16 && kTile2Y == 16, "box_xy_inplace sums 16x16 tiles");
     constexpr int NC = MOVI ? 7 : 9;
     constexpr bool CNT = !MOVI;
     // per-thread gathers / products positions at the largest radius of this variant
@@ -989,20 +1026,14 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
     const int R = a.r, W = 2 * R + 1, PX = kTile2X + 2 * R, PY = kTile2Y + 2 * R, NP = PX * PY;
     const int IXE = PX + 2, IYE = PY + 2;
     const int PS = lds_stride_2d(PX);                         // products row stride
-    constexpr int XS = lds_stride_2d(kTile2X + 1);            // x-pass row stride
     const int wex = MOVI ? IXE : PX;                          // warped region edge
     const int wofs = MOVI ? R + 1 : R;                        // it starts at x0 - wofs
     const int NI = MOVI ? IXE * IYE : NP;
-    // The x-pass output s_xp shares its space with the warped tile s_in, which is dead
-    // once the products are stored, before the box sums start; the next frame's warped
-    // tile is written only after the box sums and the combine.
-    const int uni = max(2 * NC * PY * XS, NI);                // shared region, in floats
+    // The box sums run in place in the products planes (box_xy_inplace).
     double* s_prod = s_dyn;                                   // NC*PY*PS
     double* s_red = s_prod + NC * PY * PS;                    // 8
-    double* s_xp = s_red + 8;                                 // NC*PY*XS
-    float* s_in = reinterpret_cast<float*>(s_xp);             // NI
-    int* s_cnt = reinterpret_cast<int*>(s_in + uni);          // PY*PS
-    int* s_xcnt = s_cnt + PY * PS;                            // PY*XS
+    float* s_in = reinterpret_cast<float*>(s_red + 8);        // NI
+    int* s_cnt = reinterpret_cast<int*>(s_in + NI);           // PY*PS
 
     int tile, pose;
     if (!map_block(a.tiles, a.npose, tile, pose)) return;  // whole work-group
@@ -1150,17 +1181,17 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
         // the next frame's gathers fly during the box sums and the combine
         if (f + 1 < f1) issue_frame(f + 1);
         // 3. box sums
-        box_xy_r<NC, CNT, kTile2X, kTile2Y, kThreads, RMAX>(R, s_prod, s_cnt, s_xp, s_xcnt, s_prod,
-                                                        s_cnt, tid);
+        box_xy_inplace_r<NC, CNT, kThreads, RMAX>(R, s_prod, s_cnt, tid);
         // 4. combine; each thread adds its pixel's values over the run's frames in order
         if (out_ok) {
+            const int so = box_row(oy, W) * PS + ox;  // this pixel's sums
             double S[NC];
 #pragma unroll
-            for (int c = 0; c < NC; ++c) S[c] = s_prod[c * kTile2X * kTile2Y + tid];
+            for (int c = 0; c < NC; ++c) S[c] = s_prod[c * PY * PS + so];
             double w, meas;
             bool ok;
             if (MOVI) ok = combine_static_u(S, mu, cyy, W * W, w, meas);
-            else ok = combine(S, s_cnt[tid], W * W, w, meas);
+            else ok = combine(S, s_cnt[so], W * W, w, meas);
             if (ok) {
                 acc_m += meas;
                 acc_w += w;
@@ -1679,15 +1710,16 @@ __global__ void k_convert(const void* in, int in_dtype, void* out, int out_f16, 
     }
 }
 
-// The dynamic LDS layout of k_fused2d: nc FP64 channels (7 for a moving high-resolution
-// image, 9 plus an integer count for a moving ultrasound).
+// The dynamic LDS layout of k_fused2d: nc FP64 products planes (7 for a moving
+// high-resolution image, 9 plus an integer count plane for a moving ultrasound), the
+// reduction scratch and the warped tile.
 constexpr size_t lds_bytes_2d(int r, bool movi) {
     const size_t nc = movi ? 7 : 9;
     const size_t PX = kTile2X + 2 * r, PY = kTile2Y + 2 * r, IXE = PX + 2, IYE = PY + 2;
-    const size_t PS = lds_stride_2d((int)PX), XS = lds_stride_2d(kTile2X + 1);
-    const size_t uni = std::max(2 * nc * PY * XS, movi ? IXE * IYE : PX * PY);
-    return sizeof(double) * (nc * PY * PS + 8) + sizeof(float) * uni +
-           (movi ? 0 : sizeof(int) * (PY * PS + PY * XS));
+    const size_t PS = lds_stride_2d((int)PX);
+    const size_t NI = movi ? IXE * IYE : PX * PY;
+    return sizeof(double) * (nc * PY * PS + 8) + sizeof(float) * NI +
+           (movi ? 0 : sizeof(int) * PY * PS);
 }
 static_assert(lds_bytes_2d(kMaxR2, false) <= 160 * 1024, "2D fused kernel exceeds the 160 KiB of LDS");
 
