@@ -88,8 +88,10 @@ int lc2e_memory(lc2e_engine* e, size_t* used, size_t* budget);
  * volumes (interpolator.cu:276, :315; :1616-1627 for raw files). `storage` is LC2E_F16,
  * LC2E_F32 or LC2E_STORE_AUTO (U8 -> F16, F16 -> F16, I16 -> F16 when every value lies in
  * [-2048, 2048] (CT in HU; exact in half precision) else F32, F32 -> F32: all lossless).
- * At most 2^31 - 1 voxels, and 2^30 - 1 with F32 storage (32-bit byte offsets).
- * F32 data stored as F16 must lie within +-65504 (LC2E_ERR_INVALID otherwise). */
+ * The volume and a zeroed tail of X*Y + X + 64 voxels must fit 4 GiB (32-bit byte
+ * offsets): about 2^31 voxels with F16 storage, 2^30 with F32 (LC2E_ERR_UNSUPPORTED
+ * otherwise). Every voxel must be finite, and F32 data stored as F16 must lie within
+ * +-65504 (LC2E_ERR_INVALID otherwise). */
 int lc2e_volume_upload(lc2e_engine* e, const void* host, lc2e_dtype dtype, const lc2e_geom* geom,
                        lc2e_dtype storage, lc2e_volume** out);
 void lc2e_volume_destroy(lc2e_volume* v);

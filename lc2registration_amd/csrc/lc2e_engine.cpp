@@ -569,32 +569,42 @@ int lc2e_volume_upload(lc2e_engine* e, const void* host, lc2e_dtype dtype, const
     else if (storage == LC2E_F16) f16 = 1;
     else if (storage == LC2E_F32) f16 = 0;
     else return fail(LC2E_ERR_INVALID, "storage must be F16, F32 or AUTO");
-    // the samplers address a volume with 32-bit byte offsets
-    if (!f16 && n >= (1LL << 30))
+    // the samplers address a volume and its zeroed tail with 32-bit byte offsets
+    const long long pad = lc2e::volume_pad_elems(geom->dims[0], geom->dims[1]);
+    if ((n + pad) * (f16 ? 2 : 4) > 0xffffffffLL)
         return fail(LC2E_ERR_UNSUPPORTED,
-                    "volume of %lld voxels exceeds the 4 GiB that f32 storage can address; "
-                    "store it as F16", n);
-    if (f16 && dtype == LC2E_F32) {
-        // half storage must hold every value finitely: the samplers rely on finite voxels
+                    "volume of %lld voxels (plus a %lld-voxel zeroed tail) exceeds the 4 GiB "
+                    "that %s storage can address%s", n, pad, f16 ? "f16" : "f32",
+                    f16 ? "" : "; store it as F16");
+    // The samplers rely on finite voxels (a zero-weight corner must contribute exactly 0).
+    if (dtype == LC2E_F32) {
+        const float lim = f16 ? 65504.0f : 3.4028235e38f;
         const float* h = static_cast<const float*>(host);
         for (long long i = 0; i < n; ++i)
-            if (!(std::fabs(h[i]) <= 65504.0f))
-                return fail(LC2E_ERR_INVALID, "value %g at %lld does not fit half storage", h[i], i);
+            if (!(std::fabs(h[i]) <= lim))
+                return fail(LC2E_ERR_INVALID, "value %g at %lld is not finite%s", h[i], i,
+                            f16 ? " in half storage" : "");
+    } else if (dtype == LC2E_F16) {
+        const unsigned short* h = static_cast<const unsigned short*>(host);
+        for (long long i = 0; i < n; ++i)
+            if ((h[i] & 0x7c00u) == 0x7c00u)
+                return fail(LC2E_ERR_INVALID, "half value 0x%04x at %lld is not finite", h[i], i);
     }
     auto* v = new lc2e_volume();
     v->e = e;
     v->g = *geom;
     v->f16 = f16;
-    v->bytes = (size_t)n * (f16 ? 2 : 4) + lc2e::kVolumePad;
+    const size_t esz = f16 ? 2 : 4;
+    v->bytes = (size_t)(n + pad) * esz;
     int rc = e->alloc(&v->d, v->bytes);
     if (rc < 0) {
         delete v;
         return rc;
     }
     {
-        // zeroed tail: the samplers read voxel pairs (x, x + 1) in one load
-        hipError_t pe = hipMemsetAsync(static_cast<char*>(v->d) + (v->bytes - lc2e::kVolumePad), 0,
-                                       lc2e::kVolumePad, e->stream);
+        // zeroed tail (lc2e::volume_pad_elems): the samplers' reads past the last voxel
+        hipError_t pe = hipMemsetAsync(static_cast<char*>(v->d) + (size_t)n * esz, 0,
+                                       (size_t)pad * esz, e->stream);
         if (pe != hipSuccess) {
             e->release(v->d, v->bytes);
             delete v;

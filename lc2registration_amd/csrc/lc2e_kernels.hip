@@ -226,6 +226,24 @@ __device__ __forceinline__ void issue3(const T* __restrict__ src, const SGeom& s
     issue3v(src, s, px, py, pz, inside, p);
 }
 
+// The paired loads of the pipelined samplers always take the rows y0 + 1 and planes
+// z0 + 1, where the reference takes ceil(v) (equal to v on an integer coordinate). On an
+// integer coordinate the fraction is exactly 0 there, so fmaf(0, b - a, a) == a for the
+// finite values a resident volume holds (uploads reject non-finite voxels), up to the sign
+// of a zero: the reference's value. One row / plane past the last one lands in the zeroed
+// tail (volume_pad_elems), as do all four corners of a sample outside the source. So a
+// sample needs one 32-bit element offset: the four corners are two byte offsets (plane z0
+// and z0 + 1) from two uniform bases (rows y0 and y0 + 1), with no per-corner arithmetic.
+template <typename T>
+__device__ __forceinline__ void corners_off(const T* __restrict__ src, const SGeom& s, unsigned e0,
+                                            unsigned plane, unsigned& o0, unsigned& o1,
+                                            const char*& b0, const char*& b1) {
+    o0 = e0 * (unsigned)sizeof(T);
+    o1 = o0 + plane * (unsigned)sizeof(T);
+    b0 = reinterpret_cast<const char*>(src);
+    b1 = b0 + (size_t)s.X * sizeof(T);
+}
+
 // issue3 from the source position (px, py, pz) in mm.
 template <typename T>
 __device__ __forceinline__ void issue3v(const T* __restrict__ src, const SGeom& s, float px,
@@ -233,32 +251,37 @@ __device__ __forceinline__ void issue3v(const T* __restrict__ src, const SGeom& 
     float vx, vy, vz;
     to_voxel(s, px, py, pz, vx, vy, vz);
     const float xm = (float)(s.X - 1), ym = (float)(s.Y - 1), zm = (float)(s.Z - 1);
-    const bool ok =
-        inside && vx >= 0.0f && vx <= xm && vy >= 0.0f && vy <= ym && vz >= 0.0f && vz <= zm;
-    vx = fminf(fmaxf(vx, 0.0f), xm);
-    vy = fminf(fmaxf(vy, 0.0f), ym);
-    vz = fminf(fmaxf(vz, 0.0f), zm);
-    const float fx = floorf(vx), fy = floorf(vy), fz = floorf(vz);
-    const unsigned x0 = (unsigned)fx, y0 = (unsigned)fy, z0 = (unsigned)fz;
-    const unsigned y1 = (unsigned)ceilf(vy), z1 = (unsigned)ceilf(vz);
-    p.rx = vx - fx;
-    p.ry = vy - fy;
-    p.rz = vz - fz;
+    // clamped coordinates; a coordinate lies in [0, dim - 1] exactly when the clamp leaves
+    // it unchanged (NaN is never unchanged)
+    const float cx = fminf(fmaxf(vx, 0.0f), xm);
+    const float cy = fminf(fmaxf(vy, 0.0f), ym);
+    const float cz = fminf(fmaxf(vz, 0.0f), zm);
+    const bool ok = inside && cx == vx && cy == vy && cz == vz;
+    const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
+    p.rx = cx - fx;
+    p.ry = cy - fy;
+    p.rz = cz - fz;
     const unsigned X = (unsigned)s.X, sxy = (unsigned)(s.X * s.Y);
     const unsigned tail = sxy * (unsigned)s.Z;  // first element of the zeroed tail
-    const unsigned r00 = x0 + y0 * X + z0 * sxy, r01 = x0 + y0 * X + z1 * sxy;
-    const unsigned r10 = x0 + y1 * X + z0 * sxy, r11 = x0 + y1 * X + z1 * sxy;
+    const unsigned e0 = ok ? (unsigned)fx + (unsigned)fy * X + (unsigned)fz * sxy : tail;
+    unsigned o0, o1;
+    const char *b0, *b1;
+    corners_off(src, s, e0, sxy, o0, o1, b0, b1);
     if constexpr (sizeof(T) == 2) {
-        const char* b = reinterpret_cast<const char*>(src);
-        __builtin_memcpy(&p.a, b + (ok ? r00 : tail) * 2u, 4);
-        __builtin_memcpy(&p.b, b + (ok ? r01 : tail) * 2u, 4);
-        __builtin_memcpy(&p.c, b + (ok ? r10 : tail) * 2u, 4);
-        __builtin_memcpy(&p.d, b + (ok ? r11 : tail) * 2u, 4);
+        __builtin_memcpy(&p.a, b0 + o0, 4);  // (y0, z0)
+        __builtin_memcpy(&p.b, b0 + o1, 4);  // (y0, z0 + 1)
+        __builtin_memcpy(&p.c, b1 + o0, 4);  // (y0 + 1, z0)
+        __builtin_memcpy(&p.d, b1 + o1, 4);  // (y0 + 1, z0 + 1)
     } else {
-        ld2_off(src, ok ? r00 : tail, p.a0, p.a1);
-        ld2_off(src, ok ? r01 : tail, p.b0, p.b1);
-        ld2_off(src, ok ? r10 : tail, p.c0, p.c1);
-        ld2_off(src, ok ? r11 : tail, p.d0, p.d1);
+        float2 f;
+        __builtin_memcpy(&f, b0 + o0, 8);
+        p.a0 = f.x, p.a1 = f.y;
+        __builtin_memcpy(&f, b0 + o1, 8);
+        p.b0 = f.x, p.b1 = f.y;
+        __builtin_memcpy(&f, b1 + o0, 8);
+        p.c0 = f.x, p.c1 = f.y;
+        __builtin_memcpy(&f, b1 + o1, 8);
+        p.d0 = f.x, p.d1 = f.y;
     }
 }
 
@@ -359,23 +382,27 @@ __device__ __forceinline__ void issue2(const T* __restrict__ src, const SGeom& s
         vy = (py - s.oy) / s.sy;
     }
     const float xm = (float)(s.X - 1), ym = (float)(s.Y - 1);
-    const bool ok = inside && vx >= 0.0f && vx <= xm && vy >= 0.0f && vy <= ym;
-    vx = fminf(fmaxf(vx, 0.0f), xm);
-    vy = fminf(fmaxf(vy, 0.0f), ym);
-    const float fx = floorf(vx), fy = floorf(vy);
-    const unsigned x0 = (unsigned)fx, y0 = (unsigned)fy, y1 = (unsigned)ceilf(vy);
-    p.rx = vx - fx;
-    p.ry = vy - fy;
+    const float cx = fminf(fmaxf(vx, 0.0f), xm);
+    const float cy = fminf(fmaxf(vy, 0.0f), ym);
+    const bool ok = inside && cx == vx && cy == vy;
+    const float fx = floorf(cx), fy = floorf(cy);
+    p.rx = cx - fx;
+    p.ry = cy - fy;
     const unsigned X = (unsigned)s.X;
     const unsigned tail = X * (unsigned)s.Y * (unsigned)s.Z;  // first element of the zeroed tail
-    const unsigned r0 = ok ? x0 + y0 * X : tail, r1 = ok ? x0 + y1 * X : tail;
+    // rows y0 and y0 + 1 (see corners_off): one offset from two uniform bases
+    const unsigned o = (ok ? (unsigned)fx + (unsigned)fy * X : tail) * (unsigned)sizeof(T);
+    const char* b0 = reinterpret_cast<const char*>(src);
+    const char* b1 = b0 + (size_t)s.X * sizeof(T);
     if constexpr (sizeof(T) == 2) {
-        const char* b = reinterpret_cast<const char*>(src);
-        __builtin_memcpy(&p.a, b + r0 * 2u, 4);
-        __builtin_memcpy(&p.b, b + r1 * 2u, 4);
+        __builtin_memcpy(&p.a, b0 + o, 4);
+        __builtin_memcpy(&p.b, b1 + o, 4);
     } else {
-        ld2_off(src, r0, p.a0, p.a1);
-        ld2_off(src, r1, p.b0, p.b1);
+        float2 f;
+        __builtin_memcpy(&f, b0 + o, 8);
+        p.a0 = f.x, p.a1 = f.y;
+        __builtin_memcpy(&f, b1 + o, 8);
+        p.b0 = f.x, p.b1 = f.y;
     }
 }
 

@@ -13,8 +13,11 @@ constexpr int kTile2Y = 16;    // one thread per output
 constexpr int kThreads = kTile2X * kTile2Y;  // work-group size of the 2D fused kernel
 constexpr int kTile3X = 16;    // output tile of the 3D fused kernel: 16 x 16 columns,
 constexpr int kTile3Y = 16;    // one work-group of 256 threads (two per CU), one column each
-// Resident volumes carry this many zeroed bytes past their last voxel (paired x loads).
-constexpr size_t kVolumePad = 64;
+// Resident volumes carry this many zeroed elements past their last voxel: the pipelined
+// samplers read the voxel pair (x0, x0 + 1) of rows y0 and y0 + 1 in planes z0 and z0 + 1,
+// and send samples outside the source to the first tail element. The farthest element
+// read is X*Y + X + 1 past the last voxel's successor; 64 more keep the end aligned.
+constexpr long long volume_pad_elems(long long X, long long Y) { return X * Y + X + 64; }
 constexpr int kMaxR3 = 4;      // largest patch radius of the 3D fused kernel
 constexpr int kZBlock = 8;     // 3D partial sums per block of 8 output planes; z-chunks are multiples
 constexpr int kXcds = 8;       // MI355X: 8 XCDs, each with its own L2, 32 CUs each

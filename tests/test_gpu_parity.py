@@ -254,6 +254,23 @@ def test_upload_rejects_f32_volume_beyond_32_bit_offsets(engine):
     assert b"F16" in E.lib().lc2e_last_error()
 
 
+def test_upload_rejects_non_finite_voxels(engine):
+    """The samplers rely on finite voxels (a corner of weight 0 contributes exactly 0):
+    NaN or infinite values are refused at upload, as f32 or half input, before any
+    allocation."""
+    used0, _ = engine.memory()
+    for bad in (np.nan, np.inf, -np.inf):
+        vol = np.ones((6, 5, 4), np.float32)
+        vol[2, 3, 1] = bad
+        for dt in (np.float32, np.float16):
+            with pytest.raises(E.LC2Error) as ei:
+                engine.upload(vol.astype(dt), (1, 1, 1), (0, 0, 0))
+            assert ei.value.code == -1 and b"finite" in E.lib().lc2e_last_error()
+    assert engine.memory()[0] == used0
+    ok = engine.upload(np.ones((6, 5, 4), np.float32), (1, 1, 1), (0, 0, 0))
+    ok.close()
+
+
 def test_budget_rejects_oversized_pair(w3):
     e = E.Engine(0, hbm_budget=200_000)
     with pytest.raises(E.LC2Error) as ei:
@@ -371,7 +388,8 @@ def test_ct_int16_half_storage_is_lossless(engine, wct):
     u1, _ = engine.memory()
     mv32 = engine.upload(ct.data, ct.spacing, ct.origin, E.F32)
     u2, _ = engine.memory()
-    assert (u2 - u1) - (u1 - u0) == 2 * n                            # 2 B vs 4 B per voxel
+    pad = ct.dims[0] * ct.dims[1] + ct.dims[0] + 64                   # zeroed tail, in voxels
+    assert (u2 - u1) - (u1 - u0) == 2 * (n + pad)                    # 2 B vs 4 B per voxel
     hot = ct.data.copy()
     hot.flat[n // 2] = 3000
     mvh = engine.upload(hot, ct.spacing, ct.origin)                  # AUTO, out of range
