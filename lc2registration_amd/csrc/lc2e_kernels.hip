@@ -226,6 +226,13 @@ __device__ __forceinline__ void issue3(const T* __restrict__ src, const SGeom& s
     issue3v(src, s, px, py, pz, inside, p);
 }
 
+// v clamped to [0, hi] in one instruction (fminf / fmaxf cost four: IEEE mode makes each
+// quieten its inputs first). The coordinates here come from arithmetic, so a NaN is quiet,
+// and v_med3_f32 then returns the minimum of the other two operands, 0.
+__device__ __forceinline__ float clamp_med3(float v, float hi) {
+    return __builtin_amdgcn_fmed3f(v, 0.0f, hi);
+}
+
 // The paired loads of the pipelined samplers always take the rows y0 + 1 and planes
 // z0 + 1, where the reference takes ceil(v) (equal to v on an integer coordinate). On an
 // integer coordinate the fraction is exactly 0 there, so fmaf(0, b - a, a) == a for the
@@ -251,11 +258,10 @@ __device__ __forceinline__ void issue3v(const T* __restrict__ src, const SGeom& 
     float vx, vy, vz;
     to_voxel(s, px, py, pz, vx, vy, vz);
     const float xm = (float)(s.X - 1), ym = (float)(s.Y - 1), zm = (float)(s.Z - 1);
-    // clamped coordinates; a coordinate lies in [0, dim - 1] exactly when the clamp leaves
-    // it unchanged (NaN is never unchanged)
-    const float cx = fminf(fmaxf(vx, 0.0f), xm);
-    const float cy = fminf(fmaxf(vy, 0.0f), ym);
-    const float cz = fminf(fmaxf(vz, 0.0f), zm);
+    // clamped coordinates (one v_med3_f32 each; with a quiet-NaN input it returns the
+    // smaller bound, 0); a coordinate lies in [0, dim - 1] exactly when the clamp leaves it
+    // unchanged, which NaN never is
+    const float cx = clamp_med3(vx, xm), cy = clamp_med3(vy, ym), cz = clamp_med3(vz, zm);
     const bool ok = inside && cx == vx && cy == vy && cz == vz;
     const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
     p.rx = cx - fx;
@@ -382,8 +388,7 @@ __device__ __forceinline__ void issue2(const T* __restrict__ src, const SGeom& s
         vy = (py - s.oy) / s.sy;
     }
     const float xm = (float)(s.X - 1), ym = (float)(s.Y - 1);
-    const float cx = fminf(fmaxf(vx, 0.0f), xm);
-    const float cy = fminf(fmaxf(vy, 0.0f), ym);
+    const float cx = clamp_med3(vx, xm), cy = clamp_med3(vy, ym);
     const bool ok = inside && cx == vx && cy == vy;
     const float fx = floorf(cx), fy = floorf(cy);
     p.rx = cx - fx;
