@@ -550,6 +550,9 @@ int lc2e_volume_upload(lc2e_engine* e, const void* host, lc2e_dtype dtype, const
     if (!dsz) return fail(LC2E_ERR_INVALID, "unknown dtype %d", (int)dtype);
     for (int a = 0; a < 3; ++a) {
         if (geom->dims[a] <= 0) return fail(LC2E_ERR_INVALID, "dims must be positive");
+        // the samplers form row and voxel offsets with 24-bit multiplies
+        if (geom->dims[a] >= (1 << 24))
+            return fail(LC2E_ERR_UNSUPPORTED, "dimension %d exceeds 2^24 - 1", geom->dims[a]);
         if (!(geom->spacing[a] > 0.0f)) return fail(LC2E_ERR_INVALID, "spacing must be positive");
     }
     const long long n = (long long)geom->dims[0] * geom->dims[1] * geom->dims[2];
@@ -645,8 +648,12 @@ int lc2e_pair_create(lc2e_engine* e, lc2e_volume* fixed, lc2e_volume* moving,
     if (patch_radius < 1) return fail(LC2E_ERR_INVALID, "patch radius must be >= 1 (0 selects MI, unsupported)");
     if (patch_radius > lc2e::kMaxRSep)
         return fail(LC2E_ERR_UNSUPPORTED, "patch radius %d > %d", patch_radius, lc2e::kMaxRSep);
-    // radii beyond the fused kernels take the separable path
-    const bool sep = (dim == 3 && patch_radius > lc2e::kMaxR3) || (dim == 2 && patch_radius > lc2e::kMaxR2);
+    // radii beyond the fused kernels, and 3D sources whose slices hold 2^24 voxels or more
+    // (beyond the fused samplers' 24-bit offsets), take the separable path
+    const bool big_slices = moving->g.dims[2] > 1 &&
+                            (long long)moving->g.dims[0] * moving->g.dims[1] >= lc2e::kMaxFusedSlice;
+    const bool sep = (dim == 3 && patch_radius > lc2e::kMaxR3) ||
+                     (dim == 2 && patch_radius > lc2e::kMaxR2) || big_slices;
     const int src_ndim = moving->g.dims[2] > 1 ? 3 : 2;
     if (dim == 3 && (src_ndim != 3 || fixed->g.dims[2] < 1))
         return fail(LC2E_ERR_INVALID, "3D LC2 needs a 3D moving volume");

@@ -263,13 +263,20 @@ __device__ __forceinline__ void issue3v(const T* __restrict__ src, const SGeom& 
     // unchanged, which NaN never is
     const float cx = clamp_med3(vx, xm), cy = clamp_med3(vy, ym), cz = clamp_med3(vz, zm);
     const bool ok = inside && cx == vx && cy == vy && cz == vz;
-    const float fx = floorf(cx), fy = floorf(cy), fz = floorf(cz);
-    p.rx = cx - fx;
-    p.ry = cy - fy;
-    p.rz = cz - fz;
+    // on c >= 0: conversion truncates to floor(c), and v_fract_f32 gives c - floor(c) exactly
+    p.rx = __builtin_amdgcn_fractf(cx);
+    p.ry = __builtin_amdgcn_fractf(cy);
+    p.rz = __builtin_amdgcn_fractf(cz);
     const unsigned X = (unsigned)s.X, sxy = (unsigned)(s.X * s.Y);
     const unsigned tail = sxy * (unsigned)s.Z;  // first element of the zeroed tail
-    const unsigned e0 = ok ? (unsigned)fx + (unsigned)fy * X + (unsigned)fz * sxy : tail;
+    unsigned e0 = tail;
+    if (ok) {
+        const unsigned x0 = (unsigned)cx, y0 = (unsigned)cy, z0 = (unsigned)cz;
+        // 24-bit multiply-adds (full rate; a 32-bit multiply is quarter rate): dimensions
+        // are below 2^24, and so is X*Y for every source the fused kernels sample (pairs
+        // with larger slices take the separable path)
+        e0 = __umul24(z0, sxy) + (__umul24(y0, X) + x0);
+    }
     unsigned o0, o1;
     const char *b0, *b1;
     corners_off(src, s, e0, sxy, o0, o1, b0, b1);
@@ -390,13 +397,13 @@ __device__ __forceinline__ void issue2(const T* __restrict__ src, const SGeom& s
     const float xm = (float)(s.X - 1), ym = (float)(s.Y - 1);
     const float cx = clamp_med3(vx, xm), cy = clamp_med3(vy, ym);
     const bool ok = inside && cx == vx && cy == vy;
-    const float fx = floorf(cx), fy = floorf(cy);
-    p.rx = cx - fx;
-    p.ry = cy - fy;
+    p.rx = __builtin_amdgcn_fractf(cx);  // as in issue3v
+    p.ry = __builtin_amdgcn_fractf(cy);
     const unsigned X = (unsigned)s.X;
     const unsigned tail = X * (unsigned)s.Y * (unsigned)s.Z;  // first element of the zeroed tail
     // rows y0 and y0 + 1 (see corners_off): one offset from two uniform bases
-    const unsigned o = (ok ? (unsigned)fx + (unsigned)fy * X : tail) * (unsigned)sizeof(T);
+    const unsigned o =
+        (ok ? __umul24((unsigned)cy, X) + (unsigned)cx : tail) * (unsigned)sizeof(T);
     const char* b0 = reinterpret_cast<const char*>(src);
     const char* b1 = b0 + (size_t)s.X * sizeof(T);
     if constexpr (sizeof(T) == 2) {

@@ -24,6 +24,9 @@ constexpr int kXcds = 8;       // MI355X: 8 XCDs, each with its own L2, 32 CUs e
 constexpr int kMaxR2 = 10;     // largest patch radius of the 2D fused kernel
 constexpr int kMaxRSep = 64;   // largest patch radius of the separable path (larger radii)
 constexpr int kFrameRun = 16;  // frames per 2D work-group (frame stacks)
+// The fused kernels' samplers form voxel offsets with 24-bit multiplies: a 3D source whose
+// slices hold this many voxels or more is evaluated by the separable path instead.
+constexpr long long kMaxFusedSlice = 1LL << 24;
 
 struct VGeom {  // virtual grid
     int X, Y, Z;

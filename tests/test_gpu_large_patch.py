@@ -109,6 +109,34 @@ def w3_slice():
     return make
 
 
+def test_3d_source_with_big_slices_takes_separable_path(engine):
+    """A 3D source whose slices hold 2^24 voxels or more is beyond the fused samplers'
+    24-bit voxel offsets: such a pair is evaluated by the separable path at any radius (here
+    r = 2, which the fused kernel would take otherwise). LC2 against the oracle."""
+    X = Y = 4096
+    x = np.arange(X, dtype=np.float32)
+    plane = 100.0 + 40.0 * np.sin(x * 0.02)[None, :] * np.cos(x * 0.015)[:, None]
+    mov = np.stack([plane, 0.5 * plane + 30.0]).astype(np.float32)           # (Z=2, Y, X)
+    moving = synth.Image(mov, (X, Y, 2), (0.01, 0.01, 4.0), (0.0, 0.0, 0.0))
+    rng = np.random.default_rng(12)
+    us = rng.integers(0, 256, size=(8, 40, 48), dtype=np.uint8)
+    fixed = synth.Image(us, (48, 40, 8), (0.5, 0.5, 0.5), (0.0, 0.0, 0.0))
+    pose = np.r_[0.3, -0.2, 0.1, 1.0, -2.0, 0.5, 1, 1, 1, 0, 0, 0]
+    w = synth.Workload("big-slices", fixed, moving, (0.5, 0.5, 0.5), 2, True, True, 3, pose)
+    print("3D source 4096x4096x2 (f32), r=2: separable path", flush=True)
+    p, vols = _pair(engine, w, 2, 3)
+    pose2 = np.r_[-0.4, 0.3, -0.2, -1.5, 1.0, 2.0, 1.02, 0.98, 1.0, 0.01, 0.0, 0.0]
+    mats = np.stack([p.pose_to_matrix(pose), p.pose_to_matrix(pose2)])
+    batch = p.eval_batch(mats)
+    for k, M in enumerate(mats):
+        res = _oracle(w, M, 2, 3, 3)
+        assert res.n_valid > 1000
+        assert abs(batch.lc2[k] - res.lc2) < LC2_TOL, (k, batch.lc2[k], res.lc2)
+    p.close()
+    for v in vols:
+        v.close()
+
+
 def test_radius_limits(engine):
     w = synth.small3d()
     f, m = w.fixed, w.moving
