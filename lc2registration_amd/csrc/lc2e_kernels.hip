@@ -82,17 +82,43 @@ __device__ __forceinline__ void ld2(const T* p, int i, bool same, float& a, floa
     if (same) b = a;
 }
 
+// Voxel coordinates of a position in mm: (p - origin) / spacing with one rounding, as the
+// reference's ToPxl computes it (spatialMappers.cuh:143-169). The fused kernels are
+// instantiated per form, so their samplers carry no branch for it: P2 (every spacing a
+// power of two) multiplies by the exact 1/s; otherwise the IEEE division. (A division by a
+// constant as fmaf(a, RN(1/s), RN(a * zl)), exact for a checked spacing, took more time
+// on the CT workload than the division it replaced, once its guard for tiny numerators was
+// in: 12.05 vs 11.20 ms per launch, measured.)
+template <bool P2>
+__device__ __forceinline__ float to_voxel1(float p, float o, float sp, float rs) {
+    if (P2) return (p - o) * rs;
+    return (p - o) / sp;
+}
+
+template <bool P2>
+__device__ __forceinline__ void to_voxel2(const SGeom& s, float px, float py, float& vx,
+                                          float& vy) {
+    vx = to_voxel1<P2>(px, s.ox, s.sx, s.rsx);
+    vy = to_voxel1<P2>(py, s.oy, s.sy, s.rsy);
+}
+
+template <bool P2>
 __device__ __forceinline__ void to_voxel(const SGeom& s, float px, float py, float pz, float& vx,
                                          float& vy, float& vz) {
-    if (s.pow2) {  // exact: scaling by a power of two
-        vx = (px - s.ox) * s.rsx;
-        vy = (py - s.oy) * s.rsy;
-        vz = (pz - s.oz) * s.rsz;
-    } else {
-        vx = (px - s.ox) / s.sx;
-        vy = (py - s.oy) / s.sy;
-        vz = (pz - s.oz) / s.sz;
-    }
+    to_voxel2<P2>(s, px, py, vx, vy);
+    vz = to_voxel1<P2>(pz, s.oz, s.sz, s.rsz);
+}
+
+// The same with the form chosen at run time (the one-off samplers).
+__device__ __forceinline__ void to_voxel_rt(const SGeom& s, float px, float py, float pz,
+                                            float& vx, float& vy, float& vz) {
+    if (s.pow2) to_voxel<true>(s, px, py, pz, vx, vy, vz);
+    else to_voxel<false>(s, px, py, pz, vx, vy, vz);
+}
+__device__ __forceinline__ void to_voxel2_rt(const SGeom& s, float px, float py, float& vx,
+                                             float& vy) {
+    if (s.pow2) to_voxel2<true>(s, px, py, vx, vy);
+    else to_voxel2<false>(s, px, py, vx, vy);
 }
 
 // round(b - a) in f32 for the low / high halves of two half2 words: v_fma_mix_f32 with
@@ -128,7 +154,7 @@ __device__ __forceinline__ float sample3(const T* __restrict__ src, const SGeom&
     const float py = fmaf(m[6], lz, fmaf(m[5], ly, m[4] * lx)) + m[7];
     const float pz = fmaf(m[10], lz, fmaf(m[9], ly, m[8] * lx)) + m[11];
     float vx, vy, vz;
-    to_voxel(s, px, py, pz, vx, vy, vz);
+    to_voxel_rt(s, px, py, pz, vx, vy, vz);
     if (!(vx >= 0.0f && vx <= (float)(s.X - 1) && vy >= 0.0f && vy <= (float)(s.Y - 1) &&
           vz >= 0.0f && vz <= (float)(s.Z - 1)))
         return 0.0f;
@@ -213,17 +239,17 @@ __device__ __forceinline__ void ld2_off(const T* __restrict__ base, unsigned idx
     }
 }
 
-template <typename T>
+template <bool P2, typename T>
 __device__ __forceinline__ void issue3v(const T* __restrict__ src, const SGeom& s, float px,
                                         float py, float pz, bool inside, Pending3<T>& p);
 
-template <typename T>
+template <bool P2, typename T>
 __device__ __forceinline__ void issue3(const T* __restrict__ src, const SGeom& s, const float* m,
                                        float lx, float ly, float lz, bool inside, Pending3<T>& p) {
     const float px = fmaf(m[2], lz, fmaf(m[1], ly, m[0] * lx)) + m[3];
     const float py = fmaf(m[6], lz, fmaf(m[5], ly, m[4] * lx)) + m[7];
     const float pz = fmaf(m[10], lz, fmaf(m[9], ly, m[8] * lx)) + m[11];
-    issue3v(src, s, px, py, pz, inside, p);
+    issue3v<P2>(src, s, px, py, pz, inside, p);
 }
 
 // v clamped to [0, hi] in one instruction (fminf / fmaxf cost four: IEEE mode makes each
@@ -252,11 +278,11 @@ __device__ __forceinline__ void corners_off(const T* __restrict__ src, const SGe
 }
 
 // issue3 from the source position (px, py, pz) in mm.
-template <typename T>
+template <bool P2, typename T>
 __device__ __forceinline__ void issue3v(const T* __restrict__ src, const SGeom& s, float px,
                                         float py, float pz, bool inside, Pending3<T>& p) {
     float vx, vy, vz;
-    to_voxel(s, px, py, pz, vx, vy, vz);
+    to_voxel<P2>(s, px, py, pz, vx, vy, vz);
     const float xm = (float)(s.X - 1), ym = (float)(s.Y - 1), zm = (float)(s.Z - 1);
     // clamped coordinates (one v_med3_f32 each; with a quiet-NaN input it returns the
     // smaller bound, 0); a coordinate lies in [0, dim - 1] exactly when the clamp leaves it
@@ -300,13 +326,13 @@ __device__ __forceinline__ void issue3v(const T* __restrict__ src, const SGeom& 
 
 // issue3 with the in-plane part of the source coordinates given: c = m*(lx, ly, 0) as
 // issue3 forms it, so px = fmaf(m[2], lz, c[0]) + m[3] is issue3's px bit for bit.
-template <typename T>
+template <bool P2, typename T>
 __device__ __forceinline__ void issue3c(const T* __restrict__ src, const SGeom& s, const float* m,
                                         const float* c, float lz, bool inside, Pending3<T>& p) {
     const float px = fmaf(m[2], lz, c[0]) + m[3];
     const float py = fmaf(m[6], lz, c[1]) + m[7];
     const float pz = fmaf(m[10], lz, c[2]) + m[11];
-    issue3v(src, s, px, py, pz, inside, p);
+    issue3v<P2>(src, s, px, py, pz, inside, p);
 }
 
 template <typename T>
@@ -336,13 +362,7 @@ __device__ __forceinline__ float sample2(const T* __restrict__ src, const SGeom&
     const float px = fmaf(m[1], ly, m[0] * lx) + m[3];
     const float py = fmaf(m[5], ly, m[4] * lx) + m[7];
     float vx, vy;
-    if (s.pow2) {
-        vx = (px - s.ox) * s.rsx;
-        vy = (py - s.oy) * s.rsy;
-    } else {
-        vx = (px - s.ox) / s.sx;
-        vy = (py - s.oy) / s.sy;
-    }
+    to_voxel2_rt(s, px, py, vx, vy);
     if (!(vx >= 0.0f && vx <= (float)(s.X - 1) && vy >= 0.0f && vy <= (float)(s.Y - 1)))
         return 0.0f;
     const float fx = floorf(vx), fy = floorf(vy);
@@ -381,19 +401,13 @@ struct Pending2<__half> {
     float rx, ry;
 };
 
-template <typename T>
+template <bool P2, typename T>
 __device__ __forceinline__ void issue2(const T* __restrict__ src, const SGeom& s, const float* m,
                                        float lx, float ly, bool inside, Pending2<T>& p) {
     const float px = fmaf(m[1], ly, m[0] * lx) + m[3];
     const float py = fmaf(m[5], ly, m[4] * lx) + m[7];
     float vx, vy;
-    if (s.pow2) {
-        vx = (px - s.ox) * s.rsx;
-        vy = (py - s.oy) * s.rsy;
-    } else {
-        vx = (px - s.ox) / s.sx;
-        vy = (py - s.oy) / s.sy;
-    }
+    to_voxel2<P2>(s, px, py, vx, vy);
     const float xm = (float)(s.X - 1), ym = (float)(s.Y - 1);
     const float cx = clamp_med3(vx, xm), cy = clamp_med3(vy, ym);
     const bool ok = inside && cx == vx && cy == vy;
@@ -741,7 +755,7 @@ __device__ __forceinline__ void box_xy_inplace_r(int r, double* s_prod, int* s_c
 // plane, the static image values of the next products plane and the statistics of the
 // next output plane are issued early in a step and consumed in the following one.
 // ---------------------------------------------------------------------------
-template <int R, bool MOVI, typename T, int TX, int TY>
+template <int R, bool MOVI, typename T, int TX, int TY, bool P2>
 __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
     constexpr int NTHR = TX * TY;
     constexpr int W = 2 * R + 1;
@@ -824,7 +838,7 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
         const float lz = fmaf((float)zi, a.v.sz, a.v.oz);
 #pragma unroll
         for (int j = 0; j < SPT; ++j)
-            if (wave_has(j, NI)) issue3c(src, a.s, m, cxy[j], lz, zin && ((in_xy >> j) & 1u), pend[j]);
+            if (wave_has(j, NI)) issue3c<P2>(src, a.s, m, cxy[j], lz, zin && ((in_xy >> j) & 1u), pend[j]);
     };
     // finish the gathers of the plane in flight into s_in. The finishing wave runs at
     // raised priority so that every wave reaches the next barrier sooner (+0.3%, measured).
@@ -1052,7 +1066,7 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
 // work-group reduces them once into the partial of (run, tile). The run length does not
 // depend on the batch, so neither do the results.
 // ---------------------------------------------------------------------------
-template <bool MOVI, bool SRC3D, typename T, int RMAX>
+template <bool MOVI, bool SRC3D, typename T, int RMAX, bool P2>
 __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
     extern __shared__ double s_dyn[];
     static_assert(kTile2X == 16 && kTile2Y == 16, "box_xy_inplace sums 16x16 tiles");
@@ -1127,8 +1141,8 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
         for (int j = 0; j < SPT; ++j) {
             if (j * kThreads >= NI) break;  // uniform
             const bool inside = (g_in >> j) & 1u;
-            if constexpr (SRC3D) issue3(src, a.s, m, gLx[j], gLy[j], lz, inside, pend[j]);
-            else issue2(src, a.s, m, gLx[j], gLy[j], inside, pend[j]);
+            if constexpr (SRC3D) issue3<P2>(src, a.s, m, gLx[j], gLy[j], lz, inside, pend[j]);
+            else issue2<P2>(src, a.s, m, gLx[j], gLy[j], inside, pend[j]);
         }
     };
     // static image values of the products positions (U if MOVI, else I and |grad I|)
@@ -1764,32 +1778,41 @@ static_assert(lds_bytes_2d(kMaxR2, false) <= 160 * 1024, "2D fused kernel exceed
 
 // The launch helpers below either launch (occ == nullptr) or report, in *occ, how many
 // work-groups of the kernel they would launch fit on one CU at once.
-template <int R, typename T>
+template <int R, typename T, bool P2>
 hipError_t launch3(const FusedArgs& a, bool movi, dim3 grid, hipStream_t st, int* occ) {
     constexpr int nt = kTile3X * kTile3Y;
     if (occ) {
         return movi ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                          occ, k_fused3d<R, true, T, kTile3X, kTile3Y>, nt, 0)
+                          occ, k_fused3d<R, true, T, kTile3X, kTile3Y, P2>, nt, 0)
                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                          occ, k_fused3d<R, false, T, kTile3X, kTile3Y>, nt, 0);
+                          occ, k_fused3d<R, false, T, kTile3X, kTile3Y, P2>, nt, 0);
     }
-    if (movi) hipLaunchKernelGGL((k_fused3d<R, true, T, kTile3X, kTile3Y>), grid, dim3(nt), 0, st, a);
-    else hipLaunchKernelGGL((k_fused3d<R, false, T, kTile3X, kTile3Y>), grid, dim3(nt), 0, st, a);
+    if (movi) hipLaunchKernelGGL((k_fused3d<R, true, T, kTile3X, kTile3Y, P2>), grid, dim3(nt), 0, st, a);
+    else hipLaunchKernelGGL((k_fused3d<R, false, T, kTile3X, kTile3Y, P2>), grid, dim3(nt), 0, st, a);
     return hipGetLastError();
 }
 
-template <typename T>
+template <typename T, bool P2>
 hipError_t launch3r(const FusedArgs& a, bool movi, dim3 grid, hipStream_t st, int* occ) {
     switch (a.r) {
-        case 1: return launch3<1, T>(a, movi, grid, st, occ);
-        case 2: return launch3<2, T>(a, movi, grid, st, occ);
-        case 3: return launch3<3, T>(a, movi, grid, st, occ);
-        case 4: return launch3<4, T>(a, movi, grid, st, occ);
+        case 1: return launch3<1, T, P2>(a, movi, grid, st, occ);
+        case 2: return launch3<2, T, P2>(a, movi, grid, st, occ);
+        case 3: return launch3<3, T, P2>(a, movi, grid, st, occ);
+        case 4: return launch3<4, T, P2>(a, movi, grid, st, occ);
         default: return hipErrorInvalidValue;
     }
 }
 
-template <bool MOVI, bool SRC3D, typename T, int RMAX>
+// The fused kernel for the moving image's storage type and spacing form (SGeom::pow2).
+hipError_t launch3d(const FusedArgs& a, bool movi, dim3 grid, hipStream_t st, int* occ) {
+    if (a.s.pow2)
+        return a.src_f16 ? launch3r<__half, true>(a, movi, grid, st, occ)
+                         : launch3r<float, true>(a, movi, grid, st, occ);
+    return a.src_f16 ? launch3r<__half, false>(a, movi, grid, st, occ)
+                     : launch3r<float, false>(a, movi, grid, st, occ);
+}
+
+template <bool MOVI, bool SRC3D, typename T, int RMAX, bool P2>
 hipError_t launch2r(const FusedArgs& a, dim3 grid, hipStream_t st, int* occ) {
     const size_t lds = lds_bytes_2d(a.r, MOVI);
     // allow this variant's largest radius, once per process (client threads and a serving
@@ -1797,30 +1820,41 @@ hipError_t launch2r(const FusedArgs& a, dim3 grid, hipStream_t st, int* occ) {
     static std::once_flag once;
     static hipError_t set_err = hipSuccess;
     std::call_once(once, [] {
-        set_err = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused2d<MOVI, SRC3D, T, RMAX>),
+        set_err = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused2d<MOVI, SRC3D, T, RMAX, P2>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)lds_bytes_2d(RMAX, MOVI));
     });
     if (set_err != hipSuccess) return set_err;
-    if (occ) return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_fused2d<MOVI, SRC3D, T, RMAX>,
+    if (occ) return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, k_fused2d<MOVI, SRC3D, T, RMAX, P2>,
                                                                   kThreads, lds);
-    hipLaunchKernelGGL((k_fused2d<MOVI, SRC3D, T, RMAX>), grid, dim3(kThreads), lds, st, a);
+    hipLaunchKernelGGL((k_fused2d<MOVI, SRC3D, T, RMAX, P2>), grid, dim3(kThreads), lds, st, a);
     return hipGetLastError();
 }
 
 // Radius classes of the 2D kernel: fewer registers for the common small patches.
-template <bool MOVI, bool SRC3D, typename T>
+template <bool MOVI, bool SRC3D, typename T, bool P2>
 hipError_t launch2(const FusedArgs& a, dim3 grid, hipStream_t st, int* occ) {
-    if (a.r <= 3) return launch2r<MOVI, SRC3D, T, 3>(a, grid, st, occ);
-    if (a.r <= 6) return launch2r<MOVI, SRC3D, T, 6>(a, grid, st, occ);
-    return launch2r<MOVI, SRC3D, T, kMaxR2>(a, grid, st, occ);
+    if (a.r <= 3) return launch2r<MOVI, SRC3D, T, 3, P2>(a, grid, st, occ);
+    if (a.r <= 6) return launch2r<MOVI, SRC3D, T, 6, P2>(a, grid, st, occ);
+    return launch2r<MOVI, SRC3D, T, kMaxR2, P2>(a, grid, st, occ);
 }
 
-template <typename T>
+template <typename T, bool P2>
 hipError_t launch2t(const FusedArgs& a, bool movi, int src_ndim, dim3 grid, hipStream_t st, int* occ) {
     if (movi)
-        return src_ndim == 3 ? launch2<true, true, T>(a, grid, st, occ) : launch2<true, false, T>(a, grid, st, occ);
-    return src_ndim == 3 ? launch2<false, true, T>(a, grid, st, occ) : launch2<false, false, T>(a, grid, st, occ);
+        return src_ndim == 3 ? launch2<true, true, T, P2>(a, grid, st, occ)
+                             : launch2<true, false, T, P2>(a, grid, st, occ);
+    return src_ndim == 3 ? launch2<false, true, T, P2>(a, grid, st, occ)
+                         : launch2<false, false, T, P2>(a, grid, st, occ);
+}
+
+hipError_t launch2d(const FusedArgs& a, bool movi, int src_ndim, dim3 grid, hipStream_t st,
+                    int* occ) {
+    if (a.s.pow2)
+        return a.src_f16 ? launch2t<__half, true>(a, movi, src_ndim, grid, st, occ)
+                         : launch2t<float, true>(a, movi, src_ndim, grid, st, occ);
+    return a.src_f16 ? launch2t<__half, false>(a, movi, src_ndim, grid, st, occ)
+                     : launch2t<float, false>(a, movi, src_ndim, grid, st, occ);
 }
 
 }  // namespace
@@ -1858,23 +1892,17 @@ hipError_t launch_fused(const FusedArgs& a, int grid_ndim, int src_ndim, int mov
     const long long tiles8 = (a.tiles + kXcds - 1) / kXcds * (long long)kXcds;
     if (tiles8 * n_poses > 0x7fffffffLL) return hipErrorInvalidValue;
     const dim3 grid((unsigned)(tiles8 * n_poses));
-    if (grid_ndim == 3)
-        return b.src_f16 ? launch3r<__half>(b, movi, grid, st, nullptr)
-                         : launch3r<float>(b, movi, grid, st, nullptr);
+    if (grid_ndim == 3) return launch3d(b, movi, grid, st, nullptr);
     if (b.r < 1 || b.r > kMaxR2) return hipErrorInvalidValue;
-    return b.src_f16 ? launch2t<__half>(b, movi, src_ndim, grid, st, nullptr)
-                     : launch2t<float>(b, movi, src_ndim, grid, st, nullptr);
+    return launch2d(b, movi, src_ndim, grid, st, nullptr);
 }
 
 hipError_t fused_resident_per_cu(const FusedArgs& a, int grid_ndim, int src_ndim, int movi,
                                  int* per_cu) {
     *per_cu = 0;
-    if (grid_ndim == 3)
-        return a.src_f16 ? launch3r<__half>(a, movi, dim3(1), nullptr, per_cu)
-                         : launch3r<float>(a, movi, dim3(1), nullptr, per_cu);
+    if (grid_ndim == 3) return launch3d(a, movi, dim3(1), nullptr, per_cu);
     if (a.r < 1 || a.r > kMaxR2) return hipErrorInvalidValue;
-    return a.src_f16 ? launch2t<__half>(a, movi, src_ndim, dim3(1), nullptr, per_cu)
-                     : launch2t<float>(a, movi, src_ndim, dim3(1), nullptr, per_cu);
+    return launch2d(a, movi, src_ndim, dim3(1), nullptr, per_cu);
 }
 
 static dim3 grid_for(long long n) {

@@ -112,6 +112,29 @@ def test_3d_moving_highres_parity(engine, w3, r):
     p.close()
 
 
+@pytest.mark.parametrize("spacing", [0.8, 0.7, 1.93696105])
+def test_3d_warp_bit_exact_at_non_power_of_two_spacing(engine, w3, spacing):
+    """(p - origin) / spacing must be the reference's rounded division: at spacings that are
+    not powers of two the fused kernels' samplers divide (the kernel instantiation without
+    the power-of-two product). The warp and |Sobel| are bit-exact against the oracle, and
+    LC2 within 1e-9."""
+    m = w3.moving
+    mv = synth.Image(m.data, m.dims, (spacing, spacing * 1.1, spacing * 0.9), m.origin)
+    w = synth.Workload("spacing", w3.fixed, mv, w3.resampled_spacing, 2, True, True, 3,
+                       w3.true_pose)
+    print(f"3D warp at MR spacing {spacing}", flush=True)
+    p = _pair_for(engine, w)
+    rng = np.random.default_rng(21)
+    for pose in [w3.true_pose, synth.perturb(w3.true_pose, rng, 3)]:
+        M = p.pose_to_matrix(pose)
+        res, I, U, G = _oracle_eval(w, M)
+        assert np.array_equal(p.debug_fetch(E.STAGE_WARP_I, M), I)
+        assert np.array_equal(p.debug_fetch(E.STAGE_GRAD, M), G)
+        assert res.n_valid > 1000
+        assert abs(p.eval(M) - res.lc2) < LC2_TOL
+    p.close()
+
+
 def test_3d_maps_match_oracle(engine, w3):
     p = _pair_for(engine, w3)
     M = p.pose_to_matrix(w3.true_pose)
