@@ -813,21 +813,28 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
     const int nsteps = nout + 2 * R + LAG;
 
     Pending3<T> pend[SPT];
-    // A wave whose 64 positions of sample j all lie past the warped region skips that sample
-    // (its gathers and lerps): with 576 positions on 256 threads the third sample belongs to
-    // wave 0 alone. The test is wave-uniform (scalar branch).
+    // A wave whose 64 positions of slot j all lie past the region skips that slot (its
+    // gathers and lerps, or its Sobel and products). The test is wave-uniform (scalar
+    // branch).
     const int wbase = __builtin_amdgcn_readfirstlane(tid & ~63);
     auto wave_has = [&](int j, int n) { return (j + 1) * NTHR <= n || j * NTHR + wbase < n; };
+    // The partial last gather slot goes to the last waves instead of the first (with 576
+    // positions on 256 threads the third slot belongs to wave 3 alone): those are the
+    // waves the x pass leaves idle (its 7 x PY row tasks fill the first waves).
+    constexpr int GREM = NI - (SPT - 1) * NTHR;           // positions in the last slot
+    constexpr int GSKIP = NTHR - (GREM + 63) / 64 * 64;   // threads before its first wave
+    auto gpos = [&](int j) { return j < SPT - 1 ? tid + j * NTHR : j * NTHR + tid - GSKIP; };
+    auto gwave_has = [&](int j) { return j < SPT - 1 || wbase >= GSKIP; };
     // the in-plane part of each warped position's source coordinates, m*(Lx, Ly, 0), is the
     // same for every plane: computed once, in the order issue3 uses
     float cxy[SPT][3];
     unsigned in_xy = 0;
 #pragma unroll
     for (int j = 0; j < SPT; ++j) {
-        const int i = tid + j * NTHR;
+        const int i = gpos(j);
         const int lx = i % wex, ly = i / wex;
         const int gx = x0 - wofs + lx, gy = y0 - wofs + ly;
-        if (i < NI && gx >= 0 && gx < X && gy >= 0 && gy < Y) in_xy |= 1u << j;
+        if (i >= j * NTHR && i < NI && gx >= 0 && gx < X && gy >= 0 && gy < Y) in_xy |= 1u << j;
         const float Lx = fmaf((float)gx, a.v.sx, a.v.ox), Ly = fmaf((float)gy, a.v.sy, a.v.oy);
         cxy[j][0] = fmaf(m[1], Ly, m[0] * Lx);
         cxy[j][1] = fmaf(m[5], Ly, m[4] * Lx);
@@ -838,7 +845,7 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
         const float lz = fmaf((float)zi, a.v.sz, a.v.oz);
 #pragma unroll
         for (int j = 0; j < SPT; ++j)
-            if (wave_has(j, NI)) issue3c<P2>(src, a.s, m, cxy[j], lz, zin && ((in_xy >> j) & 1u), pend[j]);
+            if (gwave_has(j)) issue3c<P2>(src, a.s, m, cxy[j], lz, zin && ((in_xy >> j) & 1u), pend[j]);
     };
     // finish the gathers of the plane in flight into s_in. The finishing wave runs at
     // raised priority so that every wave reaches the next barrier sooner (+0.3%, measured).
@@ -846,10 +853,10 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
         __builtin_amdgcn_s_setprio(2);
 #pragma unroll
         for (int j = 0; j < SPT; ++j) {
-            if (!wave_has(j, NI)) continue;
-            const int i = tid + j * NTHR;
+            if (!gwave_has(j)) continue;
+            const int i = gpos(j);
             const float v = finish3(pend[j]);
-            if (i < NI) s_in[i] = v;
+            if (i >= j * NTHR && i < NI) s_in[i] = v;
         }
         __builtin_amdgcn_s_setprio(0);
     };
