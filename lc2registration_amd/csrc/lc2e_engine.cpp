@@ -489,7 +489,9 @@ double launch_share(lc2e_pair* p, int n) {
     const Tiling t = tiling_for(p, n);
     if (t.tiles <= 0 || n <= 0) return 0.0;
     const int per_cu = resident_per_cu(p);
-    const double wgs = (double)((t.tiles + kXcds - 1) / kXcds * kXcds) * n;
+    const int prun = p->sep ? 1 : pose_run(p->v, p->info.grid_ndim, p->info.src_ndim,
+                                           p->mats_per_pose(), t.tiles, n);
+    const double wgs = (double)((t.tiles + kXcds - 1) / kXcds * kXcds) * ((n + prun - 1) / prun);
     const int cus = p->e->cus > 0 ? p->e->cus : 256;
     return wgs / ((double)cus * per_cu);
 }

@@ -1095,18 +1095,26 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
     float* s_in = reinterpret_cast<float*>(s_red + 8);        // NI
     int* s_cnt = reinterpret_cast<int*>(s_in + NI);           // PY*PS
 
-    int tile, pose;
-    if (!map_block(a.tiles, a.npose, tile, pose)) return;  // whole work-group
+    // A work-group takes one tile of either a run of frames of one pose (frame stacks, 3D
+    // sources) or, with a.prun > 1 (single images of a 2D source), a run of a.prun poses:
+    // their gathers pipeline like a run's frames, and the tile's static values are loaded
+    // once. Each pose still gets one partial per tile, so the results do not depend on the
+    // run length. (With a 3D source prun is the constant 1: the frame stacks' code is not
+    // touched by the pose runs.)
+    const int prun = (!SRC3D && a.prun > 1) ? a.prun : 1;
+    int tile, prun_idx;
+    if (!map_block(a.tiles, (a.npose + prun - 1) / prun, tile, prun_idx)) return;  // whole work-group
+    const int p0 = prun_idx * prun, p1 = min(p0 + prun, a.npose);
     const int tid = threadIdx.x;
     const int cols = a.ntx * a.nty;
     const int tcol = tile % cols, run = tile / cols;
     const int f0 = run * a.zchunk, f1 = min(f0 + a.zchunk, a.v.Z);
+    const int nit = prun > 1 ? p1 - p0 : f1 - f0;  // iterations: poses or frames
     const int tx = tcol % a.ntx, ty = tcol / a.ntx;
     const int x0 = R + tx * kTile2X, y0 = R + ty * kTile2Y;
     const int X = a.v.X, Y = a.v.Y;
     const float lz = a.v.oz;
     const T* __restrict__ src = static_cast<const T*>(a.src);
-    const float* mats = a.mats + (size_t)pose * a.mats_per_pose * 12;
 
     // Per-thread positions, the same for every frame (the radius is a run-time value, so
     // the divisions are done once here, not per frame): virtual-grid coordinates of the
@@ -1139,9 +1147,12 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
     Pend pend[SPT];
     // (No wave-uniform skip of empty slots here, unlike the 3D kernel: in this kernel it
     // cost 6% on the tracked sweep, 29.09 -> 30.93 ms per launch, measured.)
-    auto issue_frame = [&](int frame) {
+    // the matrix of iteration it: pose p0 + it (pose runs) or frame f0 + it of pose p0
+    auto issue_frame = [&](int it) {
         float m[12];
-        const float* mp = mats + (a.mats_per_pose > 1 ? frame : 0) * 12;
+        const float* mp = prun > 1 ? a.mats + (size_t)(p0 + it) * a.mats_per_pose * 12
+                                   : a.mats + ((size_t)p0 * a.mats_per_pose +
+                                               (a.mats_per_pose > 1 ? f0 + it : 0)) * 12;
 #pragma unroll
         for (int k = 0; k < 12; ++k) m[k] = mp[k];
 #pragma unroll
@@ -1178,11 +1189,12 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
     };
 
     double acc_m = 0.0, acc_w = 0.0;
-    issue_frame(f0);
+    issue_frame(0);
     issue_static(f0);
     issue_stats(f0);
-    for (int f = f0; f < f1; ++f) {
-        // 1. the warped tile of frame f (raised wave priority, as in the 3D kernel)
+    for (int it = 0; it < nit; ++it) {
+        const int f = prun > 1 ? f0 : f0 + it;  // frame of this iteration
+        // 1. the warped tile of iteration it (raised wave priority, as in the 3D kernel)
         __builtin_amdgcn_s_setprio(2);
 #pragma unroll
         for (int j = 0; j < SPT; ++j) {
@@ -1201,7 +1213,7 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
             cur1[j] = pre1[j];
         }
         const double mu = mu_pre, cyy = cyy_pre;
-        if (f + 1 < f1) {
+        if (prun == 1 && f + 1 < f1) {  // the next frame's static values (pose runs: the same)
             issue_static(f + 1);
             issue_stats(f + 1);
         }
@@ -1238,8 +1250,8 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
             }
         }
         __syncthreads();  // s_in reads done; products visible
-        // the next frame's gathers fly during the box sums and the combine
-        if (f + 1 < f1) issue_frame(f + 1);
+        // the next iteration's gathers fly during the box sums and the combine
+        if (it + 1 < nit) issue_frame(it + 1);
         // 3. box sums
         box_xy_inplace_r<NC, CNT, kThreads, RMAX>(R, s_prod, s_cnt, tid);
         // 4. combine; each thread adds its pixel's values over the run's frames in order
@@ -1262,9 +1274,17 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
                 }
             }
         }
+        if (prun > 1) {  // block-uniform: this pose's partial of the tile
+            double* dst = a.partials + ((size_t)(p0 + it) * a.blocks_per_pose + (size_t)run * cols + tcol) * 2;
+            block_reduce_store<kThreads>(acc_m, acc_w, s_red, dst);
+            acc_m = 0.0;
+            acc_w = 0.0;
+        }
     }
-    double* dst = a.partials + ((size_t)pose * a.blocks_per_pose + (size_t)run * cols + tcol) * 2;
-    block_reduce_store<kThreads>(acc_m, acc_w, s_red, dst);
+    if (prun == 1) {
+        double* dst = a.partials + ((size_t)p0 * a.blocks_per_pose + (size_t)run * cols + tcol) * 2;
+        block_reduce_store<kThreads>(acc_m, acc_w, s_red, dst);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1889,6 +1909,16 @@ int fused_blocks_2d(const VGeom& v, int r, int* ntx, int* nty) {
     return *ntx * *nty * v.Z;
 }
 
+int pose_run(const VGeom& v, int grid_ndim, int src_ndim, int mats_per_pose, int tiles,
+             int n_poses) {
+    // runs of poses per work-group while at least about two work-groups per CU slot remain
+    constexpr int kMaxPoseRun = 8;
+    constexpr long long kPoseRunWGs = 2048;
+    if (grid_ndim != 2 || src_ndim != 2 || v.Z != 1 || mats_per_pose != 1) return 1;
+    const long long tiles8 = (tiles + kXcds - 1) / kXcds * (long long)kXcds;
+    return (int)std::max(1LL, std::min<long long>(kMaxPoseRun, tiles8 * n_poses / kPoseRunWGs));
+}
+
 hipError_t launch_fused(const FusedArgs& a, int grid_ndim, int src_ndim, int movi, int n_poses,
                         hipStream_t st) {
     if (a.tiles == 0 || n_poses <= 0) return hipSuccess;
@@ -1897,8 +1927,10 @@ hipError_t launch_fused(const FusedArgs& a, int grid_ndim, int src_ndim, int mov
     FusedArgs b = a;
     b.npose = n_poses;
     const long long tiles8 = (a.tiles + kXcds - 1) / kXcds * (long long)kXcds;
-    if (tiles8 * n_poses > 0x7fffffffLL) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)(tiles8 * n_poses));
+    b.prun = pose_run(a.v, grid_ndim, src_ndim, a.mats_per_pose, a.tiles, n_poses);
+    const long long runs = (n_poses + b.prun - 1) / b.prun;
+    if (tiles8 * runs > 0x7fffffffLL) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)(tiles8 * runs));
     if (grid_ndim == 3) return launch3d(b, movi, grid, st, nullptr);
     if (b.r < 1 || b.r > kMaxR2) return hipErrorInvalidValue;
     return launch2d(b, movi, src_ndim, grid, st, nullptr);

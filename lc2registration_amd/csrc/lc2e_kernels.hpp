@@ -74,6 +74,7 @@ struct FusedArgs {
     int mats_per_pose;
     int src_f16;             // moving image stored as half
     int npose;               // poses in this launch (set by launch_fused)
+    int prun;                // 2D single images: poses per work-group (0 or 1: one)
     // separable path (patch radii beyond the fused kernels): the moving image and its
     // |Sobel| on the grid (V floats each) and two [channels][V] f64 box-sum buffers
     float* sep_img;
@@ -90,6 +91,11 @@ int fused_blocks_2d(const VGeom& v, int r, int* ntx, int* nty);
 
 hipError_t launch_fused(const FusedArgs& a, int grid_ndim, int src_ndim, int high_res_is_moving,
                         int n_poses, hipStream_t st);
+// Poses per work-group of the 2D kernel on a single image of a 2D source (1 otherwise):
+// launch_fused runs several poses of one tile in one work-group while the launch keeps
+// enough work-groups.
+int pose_run(const VGeom& v, int grid_ndim, int src_ndim, int mats_per_pose, int tiles,
+             int n_poses);
 // Work-groups of the fused kernel a launch with these arguments would use that fit on one
 // CU at once (hipOccupancyMaxActiveBlocksPerMultiprocessor).
 hipError_t fused_resident_per_cu(const FusedArgs& a, int grid_ndim, int src_ndim, int movi,

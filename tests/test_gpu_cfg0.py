@@ -72,6 +72,27 @@ def test_cfg0_256_lc2_matches_oracle(engine, seed):
     c.close()
 
 
+@pytest.mark.parametrize("n", [64, 47, 20])
+def test_cfg0_pose_runs_equal_single_and_oracle(engine, n):
+    """Batches on a single 2D image run several poses of one tile per work-group (up to 8,
+    while the launch keeps about 2048 work-groups): 64 poses of the 256x256 workload run 8
+    per work-group, 47 in runs of 5 with a partial last run, 20 in runs of 2. Each pose's
+    LC2 equals its single evaluation bit for bit, and poses at the runs' ends match the
+    oracle."""
+    print(f"2D affine 256x256, {n} poses per launch", flush=True)
+    c = Case(engine, 1)
+    rng = np.random.default_rng(n)
+    poses = [synth.perturb(c.w.true_pose, rng, 2) for _ in range(n)]
+    mats = np.stack([c.p.pose_to_matrix(q) for q in poses])
+    batch = c.p.eval_batch(mats)
+    for k in range(n):
+        assert c.p.eval(mats[k]) == batch.lc2[k], k
+    for k in (0, 4, n - 1):
+        res = c.oracle_lc2(poses[k])
+        assert abs(batch.lc2[k] - res.lc2) < LC2_TOL, (k, batch.lc2[k], res.lc2)
+    c.close()
+
+
 def test_cfg0_registration_gpu_and_oracle_agree(engine):
     c = Case(engine, 0)
     truth = np.asarray(c.w.true_pose, np.float64)
