@@ -656,17 +656,16 @@ __device__ __forceinline__ void window_run(const V* in, int stride, V* out, int 
 struct NoTail {
     __device__ void operator()() const {}
 };
-template <int NC, bool CNT, int TX, int TY, int NTHR, int W, int XS, bool PRIO_X = false,
-          bool PRIO_Y = false, class Tail = NoTail>
-__device__ __forceinline__ void box_xy(const double* s_prod, const int* s_cnt, double* s_xp,
-                                       int* s_xcnt, double* s_out, int* s_ocnt, int ps, int py,
-                                       int tid, Tail tail = Tail{}) {
-    static_assert(TX % 16 == 0 && TY % 16 == 0, "tile edges are multiples of 16");
+// x pass of box_xy: s_prod[c][py][ps] -> s_xp[c][py][XS] (TX outputs per row).
+template <int NC, bool CNT, int TX, int NTHR, int W, int XS, bool PRIO = false>
+__device__ __forceinline__ void box_x(const double* s_prod, const int* s_cnt, double* s_xp,
+                                      int* s_xcnt, int ps, int py, int tid) {
+    static_assert(TX % 16 == 0, "tile edges are multiples of 16");
     static_assert(XS > TX, "x-pass rows hold TX outputs");
     constexpr int NT = NC + (CNT ? 1 : 0);
     constexpr int SX = TX / 16;
     const int np = ps * py;
-    if constexpr (PRIO_X) __builtin_amdgcn_s_setprio(2);
+    if constexpr (PRIO) __builtin_amdgcn_s_setprio(2);
     for (int t = tid; t < NT * py * SX; t += NTHR) {
         const int row = t % py, rest = t / py;
         const int seg = rest % SX, c = rest / SX;
@@ -676,10 +675,17 @@ __device__ __forceinline__ void box_xy(const double* s_prod, const int* s_cnt, d
         else
             window_run<W>(s_cnt + row * ps + seg * 16, 1, s_xcnt + row * XS + seg * 16, 1);
     }
-    if constexpr (PRIO_X) __builtin_amdgcn_s_setprio(0);
-    __syncthreads();
+    if constexpr (PRIO) __builtin_amdgcn_s_setprio(0);
+}
+
+// y pass of box_xy: s_xp[c][py][XS] -> s_out[c][TY][TX].
+template <int NC, bool CNT, int TX, int TY, int NTHR, int W, int XS, bool PRIO = false>
+__device__ __forceinline__ void box_y(const double* s_xp, const int* s_xcnt, double* s_out,
+                                      int* s_ocnt, int py, int tid) {
+    static_assert(TY % 16 == 0, "tile edges are multiples of 16");
+    constexpr int NT = NC + (CNT ? 1 : 0);
     constexpr int SY8 = TY / 8;  // y runs of 8 outputs
-    if constexpr (PRIO_Y) __builtin_amdgcn_s_setprio(2);
+    if constexpr (PRIO) __builtin_amdgcn_s_setprio(2);
     for (int t = tid; t < NT * TX * SY8; t += NTHR) {
         const int col = t % TX, rest = t / TX;
         const int seg = rest % SY8, c = rest / SY8;
@@ -689,7 +695,17 @@ __device__ __forceinline__ void box_xy(const double* s_prod, const int* s_cnt, d
         else
             window_run<W, int, 8>(s_xcnt + seg * 8 * XS + col, XS, s_ocnt + seg * 8 * TX + col, TX);
     }
-    if constexpr (PRIO_Y) __builtin_amdgcn_s_setprio(0);
+    if constexpr (PRIO) __builtin_amdgcn_s_setprio(0);
+}
+
+template <int NC, bool CNT, int TX, int TY, int NTHR, int W, int XS, bool PRIO_X = false,
+          bool PRIO_Y = false, class Tail = NoTail>
+__device__ __forceinline__ void box_xy(const double* s_prod, const int* s_cnt, double* s_xp,
+                                       int* s_xcnt, double* s_out, int* s_ocnt, int ps, int py,
+                                       int tid, Tail tail = Tail{}) {
+    box_x<NC, CNT, TX, NTHR, W, XS, PRIO_X>(s_prod, s_cnt, s_xp, s_xcnt, ps, py, tid);
+    __syncthreads();
+    box_y<NC, CNT, TX, TY, NTHR, W, XS, PRIO_Y>(s_xp, s_xcnt, s_out, s_ocnt, py, tid);
     tail();  // work whose LDS writes the closing barrier publishes too
     __syncthreads();
 }
@@ -771,23 +787,22 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
     constexpr int LAG = MOVI ? 2 : 0;
     constexpr int NC = MOVI ? 7 : 9;
     constexpr bool CNT = !MOVI;
-    // EARLY: the next plane's gathers are finished into s_in before the y pass's closing
-    // barrier, which then also publishes s_in: three barriers per z-step instead of four
-    // (7^3 patch: 4.06 -> 3.99 ms per 32-pose launch, measured). The plane sums then need
-    // their own LDS (s_sum) instead of reusing s_prod, which the next step's products
-    // overwrite while the ring still reads; that fits two work-groups per CU with 7
-    // channels, not with 9. Not at R = 1, where the longer-lived gather registers cost a
-    // wave per SIMD (2.64 -> 3.44 ms, measured).
-    constexpr bool EARLY = MOVI && R >= 2;
+    // PIPE: steps overlap by half a step (two barriers per z-step instead of four; the
+    // loop below says how). The plane sums then need their own LDS (s_sum) instead of
+    // reusing s_prod, which the next step's products overwrite while the ring still
+    // reads; that fits two work-groups per CU with 7 channels, not with 9. Not at R = 1,
+    // where the longer-lived gather registers cost a wave per SIMD (2.64 -> 3.44 ms with
+    // the gathers finished a phase early, measured).
+    constexpr bool PIPE = MOVI && R >= 2;
 
     __shared__ float s_in[NI];
     __shared__ double s_prod[NC * PY * PS];
     __shared__ int s_cnt[CNT ? PY * PS : 1];
     __shared__ double s_xp[NC * PY * XS];
     __shared__ int s_xcnt[CNT ? PY * XS : 1];
-    __shared__ double s_sum[EARLY ? NC * TX * TY : 1];
+    __shared__ double s_sum[PIPE ? NC * TX * TY : 1];
     __shared__ double s_red[2 * NTHR / 64];
-    double* const s_out = EARLY ? s_sum : s_prod;
+    double* const s_out = PIPE ? s_sum : s_prod;
 
     int tile, pose;
     if (!map_block(a.tiles, a.npose, tile, pose)) return;  // whole work-group
@@ -912,9 +927,169 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
     issue_plane(zw0);
     issue_static(z0 - R);      // products plane of the first products step
     issue_stats(z0);           // first output plane
-    if (EARLY) {
+    if (PIPE) {
         finish_plane();
         __syncthreads();
+    }
+
+    // 2. in-plane Sobel of the 3x3 neighbourhood of the warped plane in s_in (rows: smooth
+    // s = (l + 2c) + r and difference d = r - l along x; then A, Bx, By down the rows), a
+    // 3-plane ring in registers along z, |grad| and, for q >= 0, the products of plane
+    // z0 - R + q into s_prod
+    auto sobel_products = [&](int q, const float* cur0, const float* cur1) {
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            if (!wave_has(j, NP)) continue;
+            const int i = tid + j * NTHR;
+            if (i < NP) {
+                const int lx = i % PX, ly = i / PX;
+                if (MOVI) {
+                    // rows in order, accumulating A = (s0 + 2 s1) + s2,
+                    // Bx = (d0 + 2 d1) + d2 and By = s2 - s0 as they arrive
+                    const float* row = s_in + ly * IXE + lx;
+                    float l = row[0], c = row[1], r = row[2];
+                    const float s0 = (l + 2.0f * c) + r;
+                    float a = s0, bx = r - l;
+                    row += IXE;
+                    l = row[0], c = row[1], r = row[2];
+                    a = a + 2.0f * ((l + 2.0f * c) + r);
+                    bx = bx + 2.0f * (r - l);
+                    const float cc = c;
+                    row += IXE;
+                    l = row[0], c = row[1], r = row[2];
+                    const float s2 = (l + 2.0f * c) + r;
+                    tA[j][0] = tA[j][1]; tA[j][1] = tA[j][2];
+                    tBx[j][0] = tBx[j][1]; tBx[j][1] = tBx[j][2];
+                    tBy[j][0] = tBy[j][1]; tBy[j][1] = tBy[j][2];
+                    tA[j][2] = a + s2;
+                    tBx[j][2] = bx + (r - l);
+                    tBy[j][2] = s2 - s0;
+                    Iprev[j] = Icur[j];
+                    Icur[j] = cc;
+                }
+                if (q >= 0) {
+                    float I, G, U;
+                    if (MOVI) {
+                        const float gxv = (tBx[j][0] + 2.0f * tBx[j][1]) + tBx[j][2];
+                        const float gyv = (tBy[j][0] + 2.0f * tBy[j][1]) + tBy[j][2];
+                        const float gzv = tA[j][2] - tA[j][0];
+                        G = sqrtf(fmaf(gzv, gzv, fmaf(gyv, gyv, gxv * gxv)));
+                        I = Iprev[j];
+                        U = cur0[j];
+                    } else {
+                        U = s_in[i];
+                        I = cur0[j];
+                        G = cur1[j];
+                    }
+                    store_products<NC>(s_prod, s_cnt, PY * PS, ly * PS + lx, I, G, U);
+                }
+            }
+        }
+    };
+    // 4. z ring of plane sums and combine, for the step of products plane q (q >= 0) whose
+    // plane sums are in s_out, ring slot SLOT. The patch sums add the W plane sums in z
+    // order (no running sum), so a voxel's value does not depend on where its work-group's
+    // z-chunk starts; integer counts can run.
+    auto ring_combine = [&](auto slot_c, int q) {
+        constexpr int slot = decltype(slot_c)::value;
+        const int zp = z0 - R + q;
+        if (CNT) {
+            const int cc = s_cnt[tid];
+            SC += cc - ringc[slot];
+            ringc[slot] = cc;
+        }
+        // pair sums: slot (q-1) turns from plane q-1 into Q(q-1) = P(q-1) + P(q), slot q
+        // takes P(q); a window of W = 2R+1 planes ending at q is then the R pair sums
+        // Q(q-2R), Q(q-2R+2), ..., Q(q-2) plus P(q): R+1 adds per channel, not 2R
+        constexpr int prev = (slot + W - 1) % W;
+        double Pn[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            Pn[c] = s_out[c * TX * TY + tid];
+            ring[prev][c] = ring[prev][c] + Pn[c];
+            ring[slot][c] = Pn[c];
+        }
+        if (q >= 2 * R) {
+            const double mu = mu_pre, cyy = cyy_pre;
+            if (q + 1 < nout + 2 * R) issue_stats(zp - R + 1);
+            double S[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                double t = ring[(slot + 1) % W][c];  // Q(q-2R)
+#pragma unroll
+                for (int j = 1; j < R; ++j) t += ring[(slot + 1 + 2 * j) % W][c];
+                S[c] = t + Pn[c];
+            }
+            if (col_valid) {
+                double w, meas;
+                bool ok;
+                if (MOVI) ok = combine_static_u(S, mu, cyy, N3, w, meas);
+                else ok = combine(S, SC, N3, w, meas);
+                if (ok) {
+                    acc_m += meas;
+                    acc_w += w;
+                    if (a.wmap) {
+                        const int vo = ((zp - R) * Y + (y0 + oy)) * X + (x0 + ox);
+                        a.wmap[vo] = w;
+                        a.mmap[vo] = meas;
+                    }
+                }
+            }
+            // Partial sums per block of kZBlock output planes and tile column, at fixed
+            // positions in z: every z-chunk (a multiple of kZBlock) and every batch size
+            // then adds the same numbers in the same order.
+            const int zo = zp - R;  // this step's output plane (block-uniform)
+            if ((zo - R + 1) % kZBlock == 0 || zo + 1 == zend) {
+                double* dst = a.partials + ((size_t)pose * a.blocks_per_pose +
+                                            (size_t)((zo - R) / kZBlock) * cols + tcol) * 2;
+                block_reduce_store<NTHR>(acc_m, acc_w, s_red, dst);
+                acc_m = 0.0;
+                acc_w = 0.0;
+            }
+        }
+    };
+
+    if constexpr (PIPE) {
+        // Two barriers per z-step: step s's Sobel and products share a phase with step
+        // s - 1's y pass, and step s's x pass with step s - 1's ring and combine and the
+        // finishing of plane s + 1's gathers. Each phase writes only LDS that the other
+        // phase reads, so one barrier closes it: phase 1 writes s_prod and s_sum and reads
+        // s_in and s_xp, phase 2 writes s_xp and s_in and reads s_prod and s_sum.
+        for (int s0 = 0; s0 <= nsteps; s0 += W) {
+            // W consecutive steps with compile-time ring slots, so the z ring stays in VGPRs.
+            Unroll<0, W>::run([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                const int s = s0 + k;
+                if (s > nsteps) return;  // block-uniform; step nsteps only drains s - 1
+                const int q = s - LAG;
+                const bool here = s < nsteps;
+                const bool more = s + 1 < nsteps;
+                const bool prev = s >= 1 && q >= 1;  // step s - 1 made products
+                // phase 1
+                if (here) {
+                    float cur0[PPT], cur1[PPT];
+#pragma unroll
+                    for (int j = 0; j < PPT; ++j) {
+                        cur0[j] = pre0[j];
+                        cur1[j] = pre1[j];
+                    }
+                    if (q >= 0 && q + 1 < nout + 2 * R) issue_static(z0 - R + q + 1);
+                    sobel_products(q, cur0, cur1);
+                    // the next plane's gathers fly during the y pass, the barrier and the
+                    // x pass
+                    if (more) issue_plane(zw0 + s + 1);
+                }
+                if (prev) box_y<NC, CNT, TX, TY, NTHR, W, XS, true>(s_xp, s_xcnt, s_out, s_cnt, PY, tid);
+                __syncthreads();
+                // phase 2
+                if (here && q >= 0)
+                    box_x<NC, CNT, TX, NTHR, W, XS, true>(s_prod, s_cnt, s_xp, s_xcnt, PS, PY, tid);
+                if (more) finish_plane();
+                if (prev) ring_combine(std::integral_constant<int, (k - 1 - LAG + 2 * W) % W>{}, q - 1);
+                __syncthreads();
+            });
+        }
+        return;
     }
 
     for (int s0 = 0; s0 < nsteps; s0 += W) {
@@ -925,8 +1100,8 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
             if (s >= nsteps) return;  // block-uniform
             const int q = s - LAG;
             const int zp = z0 - R + q;
-            // 1. (not EARLY) finish the plane issued last step
-            if (!EARLY) finish_plane();
+            // 1. finish the plane issued last step
+            finish_plane();
             float cur0[PPT], cur1[PPT];
 #pragma unroll
             for (int j = 0; j < PPT; ++j) {
@@ -934,132 +1109,16 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
                 cur1[j] = pre1[j];
             }
             if (q >= 0 && q + 1 < nout + 2 * R) issue_static(zp + 1);
-            if (!EARLY) __syncthreads();
-            // 2. in-plane Sobel of the 3x3 neighbourhood (rows: smooth s = (l + 2c) + r and
-            // difference d = r - l along x; then A, Bx, By down the rows), a 3-plane ring
-            // in registers along z, |grad| and the products of plane zp
-#pragma unroll
-            for (int j = 0; j < PPT; ++j) {
-                if (!wave_has(j, NP)) continue;
-                const int i = tid + j * NTHR;
-                if (i < NP) {
-                    const int lx = i % PX, ly = i / PX;
-                    if (MOVI) {
-                        // rows in order, accumulating A = (s0 + 2 s1) + s2,
-                        // Bx = (d0 + 2 d1) + d2 and By = s2 - s0 as they arrive
-                        const float* row = s_in + ly * IXE + lx;
-                        float l = row[0], c = row[1], r = row[2];
-                        const float s0 = (l + 2.0f * c) + r;
-                        float a = s0, bx = r - l;
-                        row += IXE;
-                        l = row[0], c = row[1], r = row[2];
-                        a = a + 2.0f * ((l + 2.0f * c) + r);
-                        bx = bx + 2.0f * (r - l);
-                        const float cc = c;
-                        row += IXE;
-                        l = row[0], c = row[1], r = row[2];
-                        const float s2 = (l + 2.0f * c) + r;
-                        tA[j][0] = tA[j][1]; tA[j][1] = tA[j][2];
-                        tBx[j][0] = tBx[j][1]; tBx[j][1] = tBx[j][2];
-                        tBy[j][0] = tBy[j][1]; tBy[j][1] = tBy[j][2];
-                        tA[j][2] = a + s2;
-                        tBx[j][2] = bx + (r - l);
-                        tBy[j][2] = s2 - s0;
-                        Iprev[j] = Icur[j];
-                        Icur[j] = cc;
-                    }
-                    if (q >= 0) {
-                        float I, G, U;
-                        if (MOVI) {
-                            const float gxv = (tBx[j][0] + 2.0f * tBx[j][1]) + tBx[j][2];
-                            const float gyv = (tBy[j][0] + 2.0f * tBy[j][1]) + tBy[j][2];
-                            const float gzv = tA[j][2] - tA[j][0];
-                            G = sqrtf(fmaf(gzv, gzv, fmaf(gyv, gyv, gxv * gxv)));
-                            I = Iprev[j];
-                            U = cur0[j];
-                        } else {
-                            U = s_in[i];
-                            I = cur0[j];
-                            G = cur1[j];
-                        }
-                        store_products<NC>(s_prod, s_cnt, PY * PS, ly * PS + lx, I, G, U);
-                    }
-                }
-            }
+            __syncthreads();
+            sobel_products(q, cur0, cur1);
             __syncthreads();  // s_in reads done; products visible
-            // the next plane's gathers fly during the box sums (and, not EARLY, the combine)
-            const bool more = s + 1 < nsteps;
-            if (more) issue_plane(zw0 + s + 1);
-            if (q < 0) {  // block-uniform: Sobel ring warm-up only
-                if (EARLY) {
-                    if (more) finish_plane();
-                    __syncthreads();
-                }
-                return;
-            }
-            // 3. in-plane box sums (EARLY: then the next plane into s_in)
-            box_xy<NC, CNT, TX, TY, NTHR, W, XS, true, true>(
-                s_prod, s_cnt, s_xp, s_xcnt, s_out, s_cnt, PS, PY, tid, [&]() {
-                    if (EARLY && more) finish_plane();
-                });
-            // 4. z ring of plane sums and combine. The patch sums add the W plane sums in
-            // z order (no running sum), so a voxel's value does not depend on where its
-            // work-group's z-chunk starts; integer counts can run.
-            constexpr int slot = (k - LAG + W) % W;  // ring slot of the plane stored now
-            if (CNT) {
-                const int cc = s_cnt[tid];
-                SC += cc - ringc[slot];
-                ringc[slot] = cc;
-            }
-            // pair sums: slot (q-1) turns from plane q-1 into Q(q-1) = P(q-1) + P(q), slot q
-            // takes P(q); a window of W = 2R+1 planes ending at q is then the R pair sums
-            // Q(q-2R), Q(q-2R+2), ..., Q(q-2) plus P(q): R+1 adds per channel, not 2R
-            constexpr int prev = (slot + W - 1) % W;
-            double Pn[NC];
-#pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                Pn[c] = s_out[c * TX * TY + tid];
-                ring[prev][c] = ring[prev][c] + Pn[c];
-                ring[slot][c] = Pn[c];
-            }
-            if (q >= 2 * R) {
-                const double mu = mu_pre, cyy = cyy_pre;
-                if (q + 1 < nout + 2 * R) issue_stats(zp - R + 1);
-                double S[NC];
-#pragma unroll
-                for (int c = 0; c < NC; ++c) {
-                    double t = ring[(slot + 1) % W][c];  // Q(q-2R)
-#pragma unroll
-                    for (int j = 1; j < R; ++j) t += ring[(slot + 1 + 2 * j) % W][c];
-                    S[c] = t + Pn[c];
-                }
-                if (col_valid) {
-                    double w, meas;
-                    bool ok;
-                    if (MOVI) ok = combine_static_u(S, mu, cyy, N3, w, meas);
-                    else ok = combine(S, SC, N3, w, meas);
-                    if (ok) {
-                        acc_m += meas;
-                        acc_w += w;
-                        if (a.wmap) {
-                            const int vo = ((zp - R) * Y + (y0 + oy)) * X + (x0 + ox);
-                            a.wmap[vo] = w;
-                            a.mmap[vo] = meas;
-                        }
-                    }
-                }
-                // Partial sums per block of kZBlock output planes and tile column, at
-                // fixed positions in z: every z-chunk (a multiple of kZBlock) and every
-                // batch size then adds the same numbers in the same order.
-                const int zo = zp - R;  // this step's output plane (block-uniform)
-                if ((zo - R + 1) % kZBlock == 0 || zo + 1 == zend) {
-                    double* dst = a.partials + ((size_t)pose * a.blocks_per_pose +
-                                                (size_t)((zo - R) / kZBlock) * cols + tcol) * 2;
-                    block_reduce_store<NTHR>(acc_m, acc_w, s_red, dst);
-                    acc_m = 0.0;
-                    acc_w = 0.0;
-                }
-            }
+            // the next plane's gathers fly during the box sums and the combine
+            if (s + 1 < nsteps) issue_plane(zw0 + s + 1);
+            if (q < 0) return;  // block-uniform: Sobel ring warm-up only
+            // 3. in-plane box sums
+            box_xy<NC, CNT, TX, TY, NTHR, W, XS, true, true>(s_prod, s_cnt, s_xp, s_xcnt, s_out,
+                                                            s_cnt, PS, PY, tid);
+            ring_combine(std::integral_constant<int, (k - LAG + W) % W>{}, q);
         });
     }
 }
