@@ -650,9 +650,10 @@ __device__ __forceinline__ void window_run(const V* in, int stride, V* out, int 
 // Rows (x pass) are cut into runs of 16 outputs and columns (y pass) into runs of 8, one
 // task each (runs of 8 put twice as many threads to work in the y pass); neighbouring
 // lanes take neighbouring rows (x pass) or columns (y pass).
-// PRIO_X / PRIO_Y: run the x / y pass at raised wave priority. Both fused kernels set them:
-// with several work-groups per CU out of phase, the LDS passes gain from going first
-// (3D +1.3%, 2D +3.2%, measured).
+// PRIO_X / PRIO_Y: run the x / y pass at raised wave priority. With several work-groups
+// per CU out of phase, the LDS passes gain from going first (2D +3.2%, 3D +1.3% with
+// three barriers per z-step, measured). The 3D kernel's half-step overlap keeps only the
+// y pass raised (x pass raised too: 0.5% slower, measured).
 struct NoTail {
     __device__ void operator()() const {}
 };
@@ -1083,7 +1084,7 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
                 __syncthreads();
                 // phase 2
                 if (here && q >= 0)
-                    box_x<NC, CNT, TX, NTHR, W, XS, true>(s_prod, s_cnt, s_xp, s_xcnt, PS, PY, tid);
+                    box_x<NC, CNT, TX, NTHR, W, XS, false>(s_prod, s_cnt, s_xp, s_xcnt, PS, PY, tid);
                 if (more) finish_plane();
                 if (prev) ring_combine(std::integral_constant<int, (k - 1 - LAG + 2 * W) % W>{}, q - 1);
                 __syncthreads();
