@@ -1086,7 +1086,11 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
                 if (here && q >= 0)
                     box_x<NC, CNT, TX, NTHR, W, XS, false>(s_prod, s_cnt, s_xp, s_xcnt, PS, PY, tid);
                 if (more) finish_plane();
+                // the ring and combine at raised priority: 0.7% faster (measured); the x pass
+                // stays at normal priority
+                __builtin_amdgcn_s_setprio(1);
                 if (prev) ring_combine(std::integral_constant<int, (k - 1 - LAG + 2 * W) % W>{}, q - 1);
+                __builtin_amdgcn_s_setprio(0);
                 __syncthreads();
             });
         }
