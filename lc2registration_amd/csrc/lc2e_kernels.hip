@@ -878,13 +878,33 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
     };
     // static image values (U if MOVI, else I and |grad I|) of the products positions
     float pre0[PPT], pre1[PPT];
+    // PAIR: with the moving image warped here and two products slots per thread, a thread
+    // takes two vertically adjacent positions of one column (rows ly0 and ly0 + 1, ly0
+    // even) instead of positions tid and tid + NTHR: their 3x3 neighbourhoods share two
+    // rows, so 4 rows of the warped plane are read and smoothed instead of 6, with the
+    // same operations per position.
+    constexpr int NPAIR = PX * (PY / 2);
+    constexpr bool PAIR = MOVI && PY % 2 == 0 && (NPAIR + NTHR - 1) / NTHR * 2 == PPT;
+    // products position (lx, ly) of slot j; false past the region
+    auto ppos = [&](int j, int& lx, int& ly) {
+        if (PAIR) {
+            const int t = tid + (j >> 1) * NTHR;
+            lx = t % PX;
+            ly = 2 * (t / PX) + (j & 1);
+            return t < NPAIR;
+        }
+        const int i = tid + j * NTHR;
+        lx = i % PX;
+        ly = i / PX;
+        return i < NP;
+    };
     auto issue_static = [&](int zp) {
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
-            const int i = tid + j * NTHR;
-            const int lx = i % PX, ly = i / PX;
+            int lx, ly;
+            const bool pin = ppos(j, lx, ly);
             const int gx = x0 - R + lx, gy = y0 - R + ly;
-            const bool in = i < NP && gx < X && gy < Y && zp >= 0 && zp < Z;
+            const bool in = pin && gx < X && gy < Y && zp >= 0 && zp < Z;
             const int vidx = (zp * Y + gy) * X + gx;
             pre0[j] = in ? a.stat_img[vidx] : 0.0f;
             if (!MOVI) pre1[j] = in ? a.stat_grad[vidx] : 0.0f;
@@ -938,6 +958,49 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
     // 3-plane ring in registers along z, |grad| and, for q >= 0, the products of plane
     // z0 - R + q into s_prod
     auto sobel_products = [&](int q, const float* cur0, const float* cur1) {
+        if constexpr (PAIR) {
+#pragma unroll
+            for (int jp = 0; jp < PPT / 2; ++jp) {
+                if (!wave_has(jp, NPAIR)) continue;
+                const int t = tid + jp * NTHR;
+                if (t < NPAIR) {
+                    const int lx = t % PX, ly0 = 2 * (t / PX);
+                    // rows ly0 .. ly0 + 3: smooth s = (l + 2c) + r, difference d = r - l
+                    const float* row = s_in + ly0 * IXE + lx;
+                    float sm[4], df[4], cm[4];
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr) {
+                        const float l = row[rr * IXE], c = row[rr * IXE + 1], r = row[rr * IXE + 2];
+                        sm[rr] = (l + 2.0f * c) + r;
+                        df[rr] = r - l;
+                        cm[rr] = c;
+                    }
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const int j = 2 * jp + e;
+                        // A = (s0 + 2 s1) + s2, Bx = (d0 + 2 d1) + d2, By = s2 - s0 over the
+                        // position's rows ly0 + e .. ly0 + e + 2
+                        tA[j][0] = tA[j][1]; tA[j][1] = tA[j][2];
+                        tBx[j][0] = tBx[j][1]; tBx[j][1] = tBx[j][2];
+                        tBy[j][0] = tBy[j][1]; tBy[j][1] = tBy[j][2];
+                        tA[j][2] = (sm[e] + 2.0f * sm[e + 1]) + sm[e + 2];
+                        tBx[j][2] = (df[e] + 2.0f * df[e + 1]) + df[e + 2];
+                        tBy[j][2] = sm[e + 2] - sm[e];
+                        Iprev[j] = Icur[j];
+                        Icur[j] = cm[e + 1];
+                        if (q >= 0) {
+                            const float gxv = (tBx[j][0] + 2.0f * tBx[j][1]) + tBx[j][2];
+                            const float gyv = (tBy[j][0] + 2.0f * tBy[j][1]) + tBy[j][2];
+                            const float gzv = tA[j][2] - tA[j][0];
+                            const float G = sqrtf(fmaf(gzv, gzv, fmaf(gyv, gyv, gxv * gxv)));
+                            store_products<NC>(s_prod, s_cnt, PY * PS, (ly0 + e) * PS + lx,
+                                               Iprev[j], G, cur0[j]);
+                        }
+                    }
+                }
+            }
+            return;
+        }
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
             if (!wave_has(j, NP)) continue;
