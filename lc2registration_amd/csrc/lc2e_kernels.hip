@@ -1257,14 +1257,22 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
         gLx[j] = fmaf((float)gx, a.v.sx, a.v.ox);
         gLy[j] = fmaf((float)gy, a.v.sy, a.v.oy);
     }
+    // MOVI: slots 2k and 2k + 1 are the vertically adjacent products positions (lx, ly0)
+    // and (lx, ly0 + 1), ly0 even, of pair t = tid + k * kThreads: their 3x3 Sobel
+    // neighbourhoods share two rows, so 4 rows are read and smoothed instead of 6, with the
+    // same operations per position. (PPT is even at every RMAX; PY = 16 + 2R is even.)
+    static_assert(PPT % 2 == 0, "products slots come in pairs");
+    const int NPAIR = PX * (PY / 2);
     int p_in_ofs[PPT], p_prod_ofs[PPT], p_img_ofs[PPT];  // s_in, s_prod, image (frame 0)
     unsigned p_in = 0;  // bit j: products position j holds an image value
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
-        const int i = tid + j * kThreads;
-        const int lx = i % PX, ly = i / PX;
+        const int t = tid + (j >> 1) * kThreads;
+        const int i = MOVI ? t : tid + j * kThreads;
+        const int lx = MOVI ? t % PX : i % PX;
+        const int ly = MOVI ? 2 * (t / PX) + (j & 1) : i / PX;
         const int gx = x0 - R + lx, gy = y0 - R + ly;
-        if (i < NP && gx < X && gy < Y) p_in |= 1u << j;
+        if ((MOVI ? t < NPAIR : i < NP) && gx < X && gy < Y) p_in |= 1u << j;
         p_in_ofs[j] = MOVI ? ly * IXE + lx : i;
         p_prod_ofs[j] = ly * PS + lx;
         p_img_ofs[j] = gy * X + gx;
@@ -1296,7 +1304,8 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
         const int plane = frame * X * Y;
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
-            if (j * kThreads >= NP) break;  // uniform
+            // uniform: slot j holds pair j / 2 (MOVI) or position tid + j * kThreads
+            if (MOVI ? (j >> 1) * kThreads >= NPAIR : j * kThreads >= NP) break;
             const bool in = (p_in >> j) & 1u;
             const int vidx = plane + p_img_ofs[j];
             pre0[j] = in ? a.stat_img[vidx] : 0.0f;
@@ -1346,34 +1355,39 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
         }
         __syncthreads();
         // 2. |Sobel| and the products
+        if constexpr (MOVI) {
 #pragma unroll
-        for (int j = 0; j < PPT; ++j) {
-            if (j * kThreads >= NP) break;  // uniform
-            const int i = tid + j * kThreads;
-            if (i < NP) {
-                float I, G, U;
-                if (MOVI) {
-                    // 3x3 Sobel read from the warped tile: rows smooth / difference along x
-                    float sv[3], dv[3], cc = 0.0f;
+            for (int jp = 0; jp < PPT / 2; ++jp) {
+                if (jp * kThreads >= NPAIR) break;  // uniform
+                if (tid + jp * kThreads < NPAIR) {
+                    // 3x3 Sobel read from the warped tile: rows ly0 .. ly0 + 3, smooth
+                    // s = (l + 2c) + r and difference d = r - l along x
+                    float sv[4], dv[4], cv[4];
 #pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        const float* row = s_in + p_in_ofs[j] + k * IXE;
+                    for (int k = 0; k < 4; ++k) {
+                        const float* row = s_in + p_in_ofs[2 * jp] + k * IXE;
                         const float l = row[0], c = row[1], r = row[2];
                         dv[k] = r - l;
                         sv[k] = (l + 2.0f * c) + r;
-                        if (k == 1) cc = c;
+                        cv[k] = c;
                     }
-                    const float gxv = (dv[0] + 2.0f * dv[1]) + dv[2];
-                    const float gyv = sv[2] - sv[0];
-                    G = sqrtf(fmaf(gyv, gyv, gxv * gxv));
-                    I = cc;
-                    U = cur0[j];
-                } else {
-                    U = s_in[p_in_ofs[j]];
-                    I = cur0[j];
-                    G = cur1[j];
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const float gxv = (dv[e] + 2.0f * dv[e + 1]) + dv[e + 2];
+                        const float gyv = sv[e + 2] - sv[e];
+                        const float G = sqrtf(fmaf(gyv, gyv, gxv * gxv));
+                        store_products<NC>(s_prod, s_cnt, PY * PS, p_prod_ofs[2 * jp + e],
+                                           cv[e + 1], G, cur0[2 * jp + e]);
+                    }
                 }
-                store_products<NC>(s_prod, s_cnt, PY * PS, p_prod_ofs[j], I, G, U);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < PPT; ++j) {
+                if (j * kThreads >= NP) break;  // uniform
+                if (tid + j * kThreads < NP)
+                    store_products<NC>(s_prod, s_cnt, PY * PS, p_prod_ofs[j], cur0[j], cur1[j],
+                                       s_in[p_in_ofs[j]]);
             }
         }
         __syncthreads();  // s_in reads done; products visible
