@@ -795,8 +795,20 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
     // where the longer-lived gather registers cost a wave per SIMD (2.64 -> 3.44 ms with
     // the gathers finished a phase early, measured).
     constexpr bool PIPE = MOVI && R >= 2;
+    // PAIR: with the moving image warped here and two products slots per thread, a thread
+    // takes two vertically adjacent positions of one column (rows ly0 and ly0 + 1, ly0
+    // even) instead of positions tid and tid + NTHR: their 3x3 neighbourhoods share two
+    // rows, so 4 rows of the warped plane are read and smoothed instead of 6, with the
+    // same operations per position. The warped plane's LDS rows are then SIS = 32 + PX/2
+    // floats apart: the PX lanes of a pair row read one row of it, the next PX lanes the
+    // row two below (2 SIS = 64 + PX words on), so the groups of a wave fall into
+    // distinct banks (32 or 64 banks).
+    constexpr int NPAIR = PX * (PY / 2);
+    constexpr bool PAIR = MOVI && PY % 2 == 0 && (NPAIR + NTHR - 1) / NTHR * 2 == PPT;
+    constexpr int SIS = PAIR ? 32 + PX / 2 : IXE;  // row stride of the warped plane in LDS
+    static_assert(SIS >= IXE, "warped rows fit their stride");
 
-    __shared__ float s_in[NI];
+    __shared__ float s_in[PAIR ? IYE * SIS : NI];
     __shared__ double s_prod[NC * PY * PS];
     __shared__ int s_cnt[CNT ? PY * PS : 1];
     __shared__ double s_xp[NC * PY * XS];
@@ -844,11 +856,13 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
     // the in-plane part of each warped position's source coordinates, m*(Lx, Ly, 0), is the
     // same for every plane: computed once, in the order issue3 uses
     float cxy[SPT][3];
+    int sin_ofs[SPT];  // LDS offset of warped position gpos(j) in s_in
     unsigned in_xy = 0;
 #pragma unroll
     for (int j = 0; j < SPT; ++j) {
         const int i = gpos(j);
         const int lx = i % wex, ly = i / wex;
+        sin_ofs[j] = PAIR ? ly * SIS + lx : i;
         const int gx = x0 - wofs + lx, gy = y0 - wofs + ly;
         if (i >= j * NTHR && i < NI && gx >= 0 && gx < X && gy >= 0 && gy < Y) in_xy |= 1u << j;
         const float Lx = fmaf((float)gx, a.v.sx, a.v.ox), Ly = fmaf((float)gy, a.v.sy, a.v.oy);
@@ -872,19 +886,12 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
             if (!gwave_has(j)) continue;
             const int i = gpos(j);
             const float v = finish3(pend[j]);
-            if (i >= j * NTHR && i < NI) s_in[i] = v;
+            if (i >= j * NTHR && i < NI) s_in[sin_ofs[j]] = v;
         }
         __builtin_amdgcn_s_setprio(0);
     };
     // static image values (U if MOVI, else I and |grad I|) of the products positions
     float pre0[PPT], pre1[PPT];
-    // PAIR: with the moving image warped here and two products slots per thread, a thread
-    // takes two vertically adjacent positions of one column (rows ly0 and ly0 + 1, ly0
-    // even) instead of positions tid and tid + NTHR: their 3x3 neighbourhoods share two
-    // rows, so 4 rows of the warped plane are read and smoothed instead of 6, with the
-    // same operations per position.
-    constexpr int NPAIR = PX * (PY / 2);
-    constexpr bool PAIR = MOVI && PY % 2 == 0 && (NPAIR + NTHR - 1) / NTHR * 2 == PPT;
     // products position (lx, ly) of slot j; false past the region
     auto ppos = [&](int j, int& lx, int& ly) {
         if (PAIR) {
@@ -966,11 +973,11 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
                 if (t < NPAIR) {
                     const int lx = t % PX, ly0 = 2 * (t / PX);
                     // rows ly0 .. ly0 + 3: smooth s = (l + 2c) + r, difference d = r - l
-                    const float* row = s_in + ly0 * IXE + lx;
+                    const float* row = s_in + ly0 * SIS + lx;
                     float sm[4], df[4], cm[4];
 #pragma unroll
                     for (int rr = 0; rr < 4; ++rr) {
-                        const float l = row[rr * IXE], c = row[rr * IXE + 1], r = row[rr * IXE + 2];
+                        const float l = row[rr * SIS], c = row[rr * SIS + 1], r = row[rr * SIS + 2];
                         sm[rr] = (l + 2.0f * c) + r;
                         df[rr] = r - l;
                         cm[rr] = c;
