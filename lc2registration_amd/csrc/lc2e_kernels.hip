@@ -1208,7 +1208,10 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
 // depend on the batch, so neither do the results.
 // ---------------------------------------------------------------------------
 template <bool MOVI, bool SRC3D, typename T, int RMAX, bool P2>
-__global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
+// MOVI at r <= 3: at most 128 VGPRs, four work-groups per CU (the f32-source variant
+// would take 129 and drop to three)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MOVI && RMAX <= 3 ? 4 : 1)))
+void k_fused2d(FusedArgs a) {
     extern __shared__ double s_dyn[];
     static_assert(kTile2X == 16 && kTile2Y == 16, "box_xy_inplace sums 16x16 tiles");
     constexpr int NC = MOVI ? 7 : 9;
@@ -1219,6 +1222,9 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
     constexpr int PPT = ((kTile2X + 2 * RMAX) * (kTile2Y + 2 * RMAX) + kThreads - 1) / kThreads;
     const int R = a.r, W = 2 * R + 1, PX = kTile2X + 2 * R, PY = kTile2Y + 2 * R, NP = PX * PY;
     const int IXE = PX + 2, IYE = PY + 2;
+    // MOVI: warped rows 32 + PX/2 floats apart in LDS, so the paired Sobel reads of a wave
+    // (PX lanes per row group, groups two rows apart) fall into distinct banks
+    const int SIS = MOVI ? 32 + PX / 2 : IXE;
     const int PS = lds_stride_2d(PX);                         // products row stride
     const int wex = MOVI ? IXE : PX;                          // warped region edge
     const int wofs = MOVI ? R + 1 : R;                        // it starts at x0 - wofs
@@ -1254,11 +1260,13 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
     // the divisions are done once here, not per frame): virtual-grid coordinates of the
     // warped positions, and the LDS / image offsets of the products positions.
     float gLx[SPT], gLy[SPT];
+    int g_sin[SPT];     // LDS offset of warped position j in s_in
     unsigned g_in = 0;  // bit j: warped position j lies in the grid
 #pragma unroll
     for (int j = 0; j < SPT; ++j) {
         const int i = tid + j * kThreads;
         const int lx = i % wex, ly = i / wex;
+        g_sin[j] = MOVI ? ly * SIS + lx : i;
         const int gx = x0 - wofs + lx, gy = y0 - wofs + ly;
         if (i < NI && gx >= 0 && gx < X && gy >= 0 && gy < Y) g_in |= 1u << j;
         gLx[j] = fmaf((float)gx, a.v.sx, a.v.ox);
@@ -1280,7 +1288,7 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
         const int ly = MOVI ? 2 * (t / PX) + (j & 1) : i / PX;
         const int gx = x0 - R + lx, gy = y0 - R + ly;
         if ((MOVI ? t < NPAIR : i < NP) && gx < X && gy < Y) p_in |= 1u << j;
-        p_in_ofs[j] = MOVI ? ly * IXE + lx : i;
+        p_in_ofs[j] = MOVI ? ly * SIS + lx : i;
         p_prod_ofs[j] = ly * PS + lx;
         p_img_ofs[j] = gy * X + gx;
     }
@@ -1346,7 +1354,7 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
             float v;
             if constexpr (SRC3D) v = finish3(pend[j]);
             else v = finish2(pend[j]);
-            if (i < NI) s_in[i] = v;
+            if (i < NI) s_in[g_sin[j]] = v;
         }
         __builtin_amdgcn_s_setprio(0);
         float cur0[PPT], cur1[PPT];
@@ -1372,7 +1380,7 @@ __global__ __launch_bounds__(kThreads) void k_fused2d(FusedArgs a) {
                     float sv[4], dv[4], cv[4];
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
-                        const float* row = s_in + p_in_ofs[2 * jp] + k * IXE;
+                        const float* row = s_in + p_in_ofs[2 * jp] + k * SIS;
                         const float l = row[0], c = row[1], r = row[2];
                         dv[k] = r - l;
                         sv[k] = (l + 2.0f * c) + r;
@@ -1945,7 +1953,7 @@ constexpr size_t lds_bytes_2d(int r, bool movi) {
     const size_t nc = movi ? 7 : 9;
     const size_t PX = kTile2X + 2 * r, PY = kTile2Y + 2 * r, IXE = PX + 2, IYE = PY + 2;
     const size_t PS = lds_stride_2d((int)PX);
-    const size_t NI = movi ? IXE * IYE : PX * PY;
+    const size_t NI = movi ? (32 + PX / 2) * IYE : PX * PY;
     return sizeof(double) * (nc * PY * PS + 8) + sizeof(float) * NI +
            (movi ? 0 : sizeof(int) * PY * PS);
 }
