@@ -571,6 +571,18 @@ __device__ __forceinline__ void block_reduce_store(double am, double aw, double*
     }
 }
 
+// Sums am and aw over the 64 lanes of a wave; lane 0 writes them to dst[0], dst[1].
+__device__ __forceinline__ void wave_reduce_store(double am, double aw, double* dst) {
+    for (int off = 32; off > 0; off >>= 1) {
+        am += __shfl_down(am, off, 64);
+        aw += __shfl_down(aw, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        dst[0] = am;
+        dst[1] = aw;
+    }
+}
+
 // LDS row strides of the products planes and of the x-pass output: 2 (mod 4) doubles.
 // The x pass reads and writes each lane's row two doubles at a time (ds_read_b128 /
 // ds_write2_b64), 8 lanes per LDS cycle; with a stride of 4k + 2 doubles the 8 lanes that
@@ -814,7 +826,6 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
     __shared__ double s_xp[NC * PY * XS];
     __shared__ int s_xcnt[CNT ? PY * XS : 1];
     __shared__ double s_sum[PIPE ? NC * TX * TY : 1];
-    __shared__ double s_red[2 * NTHR / 64];
     double* const s_out = PIPE ? s_sum : s_prod;
 
     int tile, pose;
@@ -1106,14 +1117,17 @@ __global__ __launch_bounds__(TX * TY, 1) void k_fused3d(FusedArgs a) {
                     }
                 }
             }
-            // Partial sums per block of kZBlock output planes and tile column, at fixed
-            // positions in z: every z-chunk (a multiple of kZBlock) and every batch size
-            // then adds the same numbers in the same order.
+            // Partial sums per block of kZBlock output planes, tile column and wave, at
+            // fixed positions in z: every z-chunk (a multiple of kZBlock) and every batch
+            // size then adds the same numbers in the same order. One partial per wave needs
+            // no barrier (a work-group sum would take one every kZBlock steps).
             const int zo = zp - R;  // this step's output plane (block-uniform)
             if ((zo - R + 1) % kZBlock == 0 || zo + 1 == zend) {
-                double* dst = a.partials + ((size_t)pose * a.blocks_per_pose +
-                                            (size_t)((zo - R) / kZBlock) * cols + tcol) * 2;
-                block_reduce_store<NTHR>(acc_m, acc_w, s_red, dst);
+                constexpr int NW = NTHR / 64;
+                double* dst = a.partials + (((size_t)pose * a.blocks_per_pose +
+                                             ((size_t)((zo - R) / kZBlock) * cols + tcol) * NW +
+                                             (tid >> 6)) * 2);
+                wave_reduce_store(acc_m, acc_w, dst);
                 acc_m = 0.0;
                 acc_w = 0.0;
             }
@@ -2054,7 +2068,7 @@ int fused_blocks_3d(const VGeom& v, int r, int zchunk, int* ntx, int* nty, int* 
 int fused_partials_3d(const VGeom& v, int r) {
     int ntx, nty, nzc;
     if (fused_blocks_3d(v, r, kZBlock, &ntx, &nty, &nzc) <= 0) return 0;
-    return ntx * nty * nzc;
+    return ntx * nty * nzc * (kTile3X * kTile3Y / 64);  // one per wave
 }
 
 int fused_blocks_2d(const VGeom& v, int r, int* ntx, int* nty) {

@@ -63,7 +63,7 @@ struct FusedArgs {
     const double* ustat_cyy;   // static ultrasound: centred patch energy, < 0 where w = 0
     const float* mats;       // [pose][mats_per_pose][12] VirtualToMoving, 3x4 row-major
     double* partials;        // [pose][blocks_per_pose][2] (sum measure, sum w); 3D: the
-                             // blocks are [z block of kZBlock planes][tile column]
+                             // blocks are [z block of kZBlock planes][tile column][wave]
     double* wmap;            // optional per-voxel w (single pose), may be null
     double* mmap;            // optional per-voxel measure, may be null
     int r;
@@ -85,7 +85,8 @@ struct FusedArgs {
 
 // Number of work-groups one pose needs (0 if the grid has no valid patch centre).
 int fused_blocks_3d(const VGeom& v, int r, int zchunk, int* ntx, int* nty, int* nzc);
-// Partial sums per pose of the 3D kernel (tile columns x blocks of kZBlock planes).
+// Partial sums per pose of the 3D kernel (tile columns x blocks of kZBlock planes x the
+// 4 waves of a work-group).
 int fused_partials_3d(const VGeom& v, int r);
 int fused_blocks_2d(const VGeom& v, int r, int* ntx, int* nty);
 
