@@ -1615,11 +1615,22 @@ __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ parti
                                                 double* __restrict__ out) {
     __shared__ double sm[256], sw[256];
     const int pose = blockIdx.x;
-    const double* p = partials + (size_t)pose * nb * 2;
+    const double2* p = reinterpret_cast<const double2*>(partials) + (size_t)pose * nb;
     double m = 0.0, w = 0.0;
-    for (int i = threadIdx.x; i < nb; i += 256) {
-        m += p[2 * i];
-        w += p[2 * i + 1];
+    // each thread adds partials tid, tid + 256, ... in that order; four loads are in
+    // flight at a time (the order of the additions, and so the result, is unchanged)
+    int i = threadIdx.x;
+    for (; i + 3 * 256 < nb; i += 4 * 256) {
+        const double2 a0 = p[i], a1 = p[i + 256], a2 = p[i + 512], a3 = p[i + 768];
+        m += a0.x; w += a0.y;
+        m += a1.x; w += a1.y;
+        m += a2.x; w += a2.y;
+        m += a3.x; w += a3.y;
+    }
+    for (; i < nb; i += 256) {
+        const double2 a = p[i];
+        m += a.x;
+        w += a.y;
     }
     sm[threadIdx.x] = m;
     sw[threadIdx.x] = w;
