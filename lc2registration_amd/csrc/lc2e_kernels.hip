@@ -666,9 +666,6 @@ __device__ __forceinline__ void window_run(const V* in, int stride, V* out, int 
 // per CU out of phase, the LDS passes gain from going first (2D +3.2%, 3D +1.3% with
 // three barriers per z-step, measured). The 3D kernel's half-step overlap keeps only the
 // y pass raised (x pass raised too: 0.5% slower, measured).
-struct NoTail {
-    __device__ void operator()() const {}
-};
 // x pass of box_xy: s_prod[c][py][ps] -> s_xp[c][py][XS] (TX outputs per row).
 template <int NC, bool CNT, int TX, int NTHR, int W, int XS, bool PRIO = false>
 __device__ __forceinline__ void box_x(const double* s_prod, const int* s_cnt, double* s_xp,
@@ -712,14 +709,13 @@ __device__ __forceinline__ void box_y(const double* s_xp, const int* s_xcnt, dou
 }
 
 template <int NC, bool CNT, int TX, int TY, int NTHR, int W, int XS, bool PRIO_X = false,
-          bool PRIO_Y = false, class Tail = NoTail>
+          bool PRIO_Y = false>
 __device__ __forceinline__ void box_xy(const double* s_prod, const int* s_cnt, double* s_xp,
                                        int* s_xcnt, double* s_out, int* s_ocnt, int ps, int py,
-                                       int tid, Tail tail = Tail{}) {
+                                       int tid) {
     box_x<NC, CNT, TX, NTHR, W, XS, PRIO_X>(s_prod, s_cnt, s_xp, s_xcnt, ps, py, tid);
     __syncthreads();
     box_y<NC, CNT, TX, TY, NTHR, W, XS, PRIO_Y>(s_xp, s_xcnt, s_out, s_ocnt, py, tid);
-    tail();  // work whose LDS writes the closing barrier publishes too
     __syncthreads();
 }
 
